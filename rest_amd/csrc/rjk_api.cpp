@@ -1,0 +1,647 @@
+// rjk_api.cpp — the C ABI declared in include/rest_jk.h: contexts, device B
+// storage, workspace, the J/K entry points and phase timing.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rjk_internal.hpp"
+
+namespace {
+thread_local std::string g_create_error;
+
+constexpr double kEigRelCut = 1e-13;     // density-only route: |λ| <= cut·max|λ| is dropped
+constexpr double kAsymTol = 1e-10;       // density-only route: D symmetry tolerance
+
+struct Carve {
+    std::vector<size_t> off;
+    size_t total = 0;
+    size_t add(size_t bytes) {
+        off.push_back(total);
+        total += rjk::round_up(bytes, 256);
+        return off.size() - 1;
+    }
+};
+}  // namespace
+
+namespace rjk {
+
+int fail(rjk_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int ws_get(rjk_ctx* ctx, size_t bytes, void** out) {
+    if (bytes > ctx->ws_size) {
+        RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->ws) (void)hipFree(ctx->ws);
+        ctx->ws = nullptr;
+        ctx->ws_size = 0;
+        size_t sz = round_up(bytes, size_t(1) << 20);
+        if (hipMalloc(&ctx->ws, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->ws = nullptr;
+            return fail(ctx, RJK_ERR_NOMEM, "workspace allocation of " + std::to_string(sz) + " bytes failed");
+        }
+        ctx->ws_size = sz;
+    }
+    *out = ctx->ws;
+    return RJK_OK;
+}
+
+PhaseScope::PhaseScope(rjk_ctx* c, int phase) : ctx(c), on(c->prof) {
+    if (!on) return;
+    if (!ctx->ev_free.empty()) {
+        ev = ctx->ev_free.back();
+        ctx->ev_free.pop_back();
+    } else {
+        (void)hipEventCreate(&ev.a);
+        (void)hipEventCreate(&ev.b);
+    }
+    ev.phase = phase;
+    (void)hipEventRecord(ev.a, ctx->stream);
+}
+PhaseScope::~PhaseScope() {
+    if (!on) return;
+    (void)hipEventRecord(ev.b, ctx->stream);
+    ctx->ev_pending.push_back(ev);
+}
+
+}  // namespace rjk
+
+using namespace rjk;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int rjk_create(int device, size_t num_basis, size_t num_auxbas, rjk_ctx** out) {
+    if (!out) return fail(nullptr, RJK_ERR_ARG, "rjk_create: out is NULL");
+    *out = nullptr;
+    if (num_basis == 0 || num_auxbas == 0)
+        return fail(nullptr, RJK_ERR_ARG, "rjk_create: num_basis and num_auxbas must be positive");
+    if (num_basis > 46340)  // nbf^2 must fit the int tile arithmetic of the K kernels
+        return fail(nullptr, RJK_ERR_ARG, "rjk_create: num_basis too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(nullptr, RJK_ERR_HIP, "rjk_create: HIP device " + std::to_string(device) + " not available");
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, RJK_ERR_HIP, "rjk_create: hipSetDevice failed");
+    rjk_ctx* c = new rjk_ctx();
+    c->device = device;
+    c->nbf = num_basis;
+    c->naux = num_auxbas;
+    c->npair = num_basis * (num_basis + 1) / 2;
+    c->ldb = round_up(c->npair, kLdAlign);
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(nullptr, RJK_ERR_HIP, "rjk_create: stream creation failed");
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return RJK_OK;
+}
+
+extern "C" int rjk_destroy(rjk_ctx* ctx) {
+    if (!ctx) return RJK_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->b64) (void)hipFree(ctx->b64);
+    if (ctx->b32) (void)hipFree(ctx->b32);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->solver) rocblas_destroy_handle((rocblas_handle)ctx->solver);
+    for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    for (auto& e : ctx->ev_free) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return RJK_OK;
+}
+
+extern "C" const char* rjk_last_error(const rjk_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+extern "C" size_t rjk_num_basis(const rjk_ctx* ctx) { return ctx ? ctx->nbf : 0; }
+extern "C" size_t rjk_num_auxbas(const rjk_ctx* ctx) { return ctx ? ctx->naux : 0; }
+extern "C" size_t rjk_num_baspar(const rjk_ctx* ctx) { return ctx ? ctx->npair : 0; }
+extern "C" size_t rjk_ld_baspar(const rjk_ctx* ctx) { return ctx ? ctx->ldb : 0; }
+
+extern "C" int rjk_set_stream(rjk_ctx* ctx, void* stream) {
+    if (!ctx) return RJK_ERR_ARG;
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    if (ctx->solver) rocblas_set_stream((rocblas_handle)ctx->solver, ctx->stream);
+    return RJK_OK;
+}
+
+extern "C" int rjk_synchronize(rjk_ctx* ctx) {
+    if (!ctx) return RJK_ERR_ARG;
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RJK_OK;
+}
+
+extern "C" int rjk_set_workspace_limit(rjk_ctx* ctx, size_t bytes) {
+    if (!ctx) return RJK_ERR_ARG;
+    ctx->ws_limit = bytes ? bytes : (size_t(4) << 30);
+    return RJK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// B storage
+// ---------------------------------------------------------------------------
+namespace {
+
+int alloc_b(rjk_ctx* ctx, unsigned mask) {
+    if (!(mask & RJK_STORE_FP64))
+        return fail(ctx, RJK_ERR_ARG, "store_mask must include RJK_STORE_FP64 (the FP64 copy is the reference)");
+    if (mask & RJK_STORE_BF16X2) return fail(ctx, RJK_ERR_ARG, "RJK_STORE_BF16X2 is not available in this build");
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t n = ctx->naux * ctx->ldb;
+    if (!ctx->b64) {
+        if (hipMalloc(&ctx->b64, n * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->b64 = nullptr;
+            return fail(ctx, RJK_ERR_NOMEM, "B (FP64) allocation of " + std::to_string(n * 8) + " bytes failed");
+        }
+    }
+    if ((mask & RJK_STORE_FP32) && !ctx->b32) {
+        if (hipMalloc(&ctx->b32, n * sizeof(float)) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->b32 = nullptr;
+            return fail(ctx, RJK_ERR_NOMEM, "B (FP32) allocation failed");
+        }
+    }
+    if (!(mask & RJK_STORE_FP32) && ctx->b32) {
+        (void)hipFree(ctx->b32);
+        ctx->b32 = nullptr;
+    }
+    return RJK_OK;
+}
+
+int finish_b(rjk_ctx* ctx) {
+    if (ctx->b32) RJK_HIP(ctx, launch_to_fp32(ctx->stream, ctx->b64, ctx->naux * ctx->ldb, ctx->b32));
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RJK_OK;
+}
+
+}  // namespace
+
+extern "C" int rjk_set_rimatr(rjk_ctx* ctx, const double* rimatr, unsigned store_mask) {
+    if (!ctx || !rimatr) return ctx ? fail(ctx, RJK_ERR_ARG, "rimatr is NULL") : RJK_ERR_ARG;
+    int rc = alloc_b(ctx, store_mask);
+    if (rc) return rc;
+    RJK_HIP(ctx, hipMemsetAsync(ctx->b64, 0, ctx->naux * ctx->ldb * sizeof(double), ctx->stream));
+    RJK_HIP(ctx, hipMemcpy2DAsync(ctx->b64, ctx->ldb * sizeof(double), rimatr, ctx->npair * sizeof(double),
+                                  ctx->npair * sizeof(double), ctx->naux, hipMemcpyHostToDevice, ctx->stream));
+    return finish_b(ctx);
+}
+
+extern "C" int rjk_set_rimatr_dev(rjk_ctx* ctx, const double* rimatr_dev, unsigned store_mask) {
+    if (!ctx || !rimatr_dev) return ctx ? fail(ctx, RJK_ERR_ARG, "rimatr_dev is NULL") : RJK_ERR_ARG;
+    int rc = alloc_b(ctx, store_mask);
+    if (rc) return rc;
+    RJK_HIP(ctx, launch_pad_rimatr(ctx->stream, rimatr_dev, ctx->npair, ctx->naux, ctx->ldb, ctx->b64));
+    return finish_b(ctx);
+}
+
+extern "C" int rjk_set_ri3fn(rjk_ctx* ctx, const double* ri3fn, unsigned store_mask) {
+    if (!ctx || !ri3fn) return ctx ? fail(ctx, RJK_ERR_ARG, "ri3fn is NULL") : RJK_ERR_ARG;
+    int rc = alloc_b(ctx, store_mask);
+    if (rc) return rc;
+    const size_t n2 = ctx->nbf * ctx->nbf;
+    // stream full columns through a staging slab of at most 256 MiB
+    size_t slab = std::max<size_t>(1, (size_t(256) << 20) / (n2 * sizeof(double)));
+    slab = std::min(slab, ctx->naux);
+    void* ws = nullptr;
+    rc = ws_get(ctx, slab * n2 * sizeof(double), &ws);
+    if (rc) return rc;
+    for (size_t P0 = 0; P0 < ctx->naux; P0 += slab) {
+        size_t nP = std::min(slab, ctx->naux - P0);
+        RJK_HIP(ctx, hipMemcpyAsync(ws, ri3fn + P0 * n2, nP * n2 * sizeof(double), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        RJK_HIP(ctx, launch_pack_ri3fn(ctx->stream, (const double*)ws, ctx->nbf, nP, ctx->ldb,
+                                       ctx->b64 + P0 * ctx->ldb));
+        RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));  // pageable source: keep the slab alive
+    }
+    return finish_b(ctx);
+}
+
+extern "C" int rjk_fill_rimatr_synthetic(rjk_ctx* ctx, uint64_t seed, unsigned store_mask) {
+    if (!ctx) return RJK_ERR_ARG;
+    int rc = alloc_b(ctx, store_mask);
+    if (rc) return rc;
+    RJK_HIP(ctx, launch_fill_synth(ctx->stream, seed, ctx->npair, ctx->naux, ctx->ldb, ctx->b64));
+    return finish_b(ctx);
+}
+
+extern "C" const double* rjk_rimatr_dev(const rjk_ctx* ctx) { return ctx ? ctx->b64 : nullptr; }
+
+// ---------------------------------------------------------------------------
+// J
+// ---------------------------------------------------------------------------
+namespace {
+
+int check_common(rjk_ctx* ctx, size_t spin_channel, int prec, const void** bptr) {
+    if (spin_channel != 1 && spin_channel != 2)
+        return fail(ctx, RJK_ERR_ARG, "spin_channel must be 1 or 2, got " + std::to_string(spin_channel));
+    if (!ctx->b64) return fail(ctx, RJK_ERR_STATE, "B is not set (rjk_set_rimatr / rjk_set_ri3fn first)");
+    if (prec == RJK_PREC_FP64) *bptr = ctx->b64;
+    else if (prec == RJK_PREC_FP32) {
+        if (!ctx->b32) return fail(ctx, RJK_ERR_STATE, "FP32 copy of B not stored (RJK_STORE_FP32)");
+        *bptr = ctx->b32;
+    } else return fail(ctx, RJK_ERR_ARG, "unsupported precision " + std::to_string(prec));
+    return RJK_OK;
+}
+
+}  // namespace
+
+extern "C" int rjk_vj_upper_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel, double scaling_factor,
+                                int prec, int full_dm, double* vj_dev) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!dm_dev || !vj_dev) return fail(ctx, RJK_ERR_ARG, "vj: NULL device pointer");
+    const void* b = nullptr;
+    int rc = check_common(ctx, spin_channel, prec, &b);
+    if (rc) return rc;
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    const int ns = (int)spin_channel;
+    size_t nchunk, nsplit;
+    j_plan(ctx->ldb, ctx->naux, prec, &nchunk, &nsplit);
+    Carve cv;
+    size_t iw = cv.add(ns * ctx->ldb * sizeof(double));
+    size_t ig = cv.add(nchunk * ns * ctx->naux * sizeof(double));
+    size_t iG = cv.add(ns * ctx->naux * sizeof(double));
+    size_t ij = cv.add(nsplit * ns * ctx->ldb * sizeof(double));
+    void* ws = nullptr;
+    rc = ws_get(ctx, cv.total, &ws);
+    if (rc) return rc;
+    char* base = (char*)ws;
+    double* w = (double*)(base + cv.off[iw]);
+    double* gpart = (double*)(base + cv.off[ig]);
+    double* gamma = (double*)(base + cv.off[iG]);
+    double* jpart = (double*)(base + cv.off[ij]);
+    hipStream_t st = ctx->stream;
+    {
+        PhaseScope ps(ctx, 4);
+        RJK_HIP(ctx, launch_dm_weights(st, dm_dev, ctx->nbf, ctx->npair, ctx->ldb, ns, full_dm, w));
+    }
+    {
+        PhaseScope ps(ctx, 0);
+        RJK_HIP(ctx, launch_jgamma(st, prec, b, ctx->ldb, ctx->naux, ns, w, nchunk, gpart));
+    }
+    {
+        PhaseScope ps(ctx, 4);
+        RJK_HIP(ctx, launch_gamma_reduce(st, gpart, nchunk, ns, ctx->naux, gamma));
+    }
+    {
+        PhaseScope ps(ctx, 1);
+        RJK_HIP(ctx, launch_jacc(st, prec, b, ctx->ldb, ctx->naux, ns, gamma, nsplit, jpart));
+    }
+    {
+        PhaseScope ps(ctx, 4);
+        RJK_HIP(ctx, launch_jreduce(st, jpart, nsplit, ns, ctx->ldb, ctx->npair, scaling_factor, vj_dev));
+    }
+    return RJK_OK;
+}
+
+namespace {
+
+// host staging buffer separate from the internal workspace
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+int check_host_finite(rjk_ctx* ctx, const double* x, size_t n, const char* what) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i])) return fail(ctx, RJK_ERR_NONFINITE, std::string(what) + " holds NaN/Inf");
+    return RJK_OK;
+}
+
+int vj_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scaling_factor, int prec,
+            double* vj_upper, int full_dm) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!dm || !vj_upper) return fail(ctx, RJK_ERR_ARG, "vj: NULL host pointer");
+    if (spin_channel != 1 && spin_channel != 2)
+        return fail(ctx, RJK_ERR_ARG, "spin_channel must be 1 or 2, got " + std::to_string(spin_channel));
+    const size_t n2 = ctx->nbf * ctx->nbf;
+    int rc = check_host_finite(ctx, dm, spin_channel * n2, "dm");
+    if (rc) return rc;
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf io;
+    const size_t out_n = spin_channel * ctx->npair;
+    RJK_HIP(ctx, hipMalloc(&io.p, (spin_channel * n2 + out_n) * sizeof(double)));
+    double* d_dm = (double*)io.p;
+    double* d_out = d_dm + spin_channel * n2;
+    RJK_HIP(ctx, hipMemcpyAsync(d_dm, dm, spin_channel * n2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    rc = rjk_vj_upper_dev(ctx, d_dm, spin_channel, scaling_factor, prec, full_dm, d_out);
+    if (rc) return rc;
+    RJK_HIP(ctx, hipMemcpyAsync(vj_upper, d_out, out_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RJK_OK;
+}
+
+}  // namespace
+
+extern "C" int rjk_vj_upper_with_rimatr_v_sync(rjk_ctx* ctx, const double* dm, size_t spin_channel,
+                                               double scaling_factor, int prec, double* vj_upper) {
+    return vj_host(ctx, dm, spin_channel, scaling_factor, prec, vj_upper, 0);
+}
+
+extern "C" int rjk_vj_upper_with_ri_v_sync(rjk_ctx* ctx, const double* dm, size_t spin_channel,
+                                           double scaling_factor, int prec, double* vj_upper) {
+    return vj_host(ctx, dm, spin_channel, scaling_factor, prec, vj_upper, 1);
+}
+
+// ---------------------------------------------------------------------------
+// K
+// ---------------------------------------------------------------------------
+namespace {
+
+void k_plan_cols(KPlan& kp, size_t nbf, const size_t* ncol, int ns) {
+    kp.nbf_pad = round_up(nbf, kTileMN);
+    kp.col_cnt[0] = round_up(ncol[0], kTileK);
+    kp.col_off[0] = 0;
+    kp.col_cnt[1] = ns == 2 ? round_up(ncol[1], kTileK) : 0;
+    kp.col_off[1] = kp.col_cnt[0];
+    kp.nk = round_up(kp.col_cnt[0] + kp.col_cnt[1], kColPad);
+}
+
+// chunking and split-K; carves [cocc | sign | kpart | M'] out of the workspace
+int k_plan_ws(rjk_ctx* ctx, KPlan& kp, int ns, double** cocc, double** sign, double** kpart, double** m) {
+    const size_t nt = kp.nbf_pad / kTileMN;
+    const size_t ntiles = nt * (nt + 1) / 2;
+    const size_t target = 1024;  // SYRK work-groups
+    const size_t slot = kp.nbf_pad * kp.nbf_pad;
+    const size_t slot_bytes = (size_t)ns * slot * sizeof(double);
+    const size_t perP = kp.nk * kp.nbf_pad * sizeof(double);
+    const size_t small = round_up(kp.nbf_pad * kp.nk * sizeof(double), 256) + round_up(kp.nk * sizeof(double), 256);
+    // split-K slots: enough work-groups, at most a quarter of the workspace,
+    // and at least 8 auxiliary columns per slot in every chunk
+    size_t nsplit = std::max<size_t>(1, (target + ntiles * ns - 1) / (ntiles * ns));
+    nsplit = std::min(nsplit, std::max<size_t>(1, (ctx->ws_limit / 4) / slot_bytes));
+    size_t chunk = 1;
+    for (int it = 0; it < 3; ++it) {
+        size_t fixed = small + nsplit * slot_bytes;
+        size_t avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
+        chunk = perP ? std::min(ctx->naux, std::max<size_t>(1, avail / perP)) : ctx->naux;
+        if (chunk > 65535) chunk = 65535;  // grid.z limit
+        size_t cap = std::max<size_t>(1, chunk / 8);
+        if (nsplit <= cap) break;
+        nsplit = cap;
+    }
+    kp.nsplit = nsplit;
+    Carve cv;
+    size_t ic = cv.add(kp.nbf_pad * kp.nk * sizeof(double));
+    size_t is = cv.add(kp.nk * sizeof(double));
+    size_t ik = cv.add(kp.nsplit * ns * slot * sizeof(double));
+    kp.chunk = chunk;
+    size_t im = cv.add(chunk * perP);
+    void* ws = nullptr;
+    int rc = ws_get(ctx, cv.total, &ws);
+    if (rc) return rc;
+    char* base = (char*)ws;
+    *cocc = (double*)(base + cv.off[ic]);
+    *sign = (double*)(base + cv.off[is]);
+    *kpart = (double*)(base + cv.off[ik]);
+    *m = (double*)(base + cv.off[im]);
+    return RJK_OK;
+}
+
+int k_run(rjk_ctx* ctx, const KPlan& kp, int ns, int prec, const void* b, const double* cocc, const double* sign,
+          double* kpart, double* m, double scale, double* vk_dev) {
+    hipStream_t st = ctx->stream;
+    if (kp.nk == 0) {  // no occupied column (or D = 0): K = 0
+        RJK_HIP(ctx, hipMemsetAsync(vk_dev, 0, (size_t)ns * ctx->npair * sizeof(double), st));
+        return RJK_OK;
+    }
+    for (size_t P0 = 0; P0 < ctx->naux; P0 += kp.chunk) {
+        size_t Pc = std::min(kp.chunk, ctx->naux - P0);
+        {
+            PhaseScope ps(ctx, 2);
+            RJK_HIP(ctx, launch_khalf(st, prec, b, ctx->ldb, ctx->nbf, P0, Pc, kp, cocc, m));
+        }
+        {
+            PhaseScope ps(ctx, 3);
+            RJK_HIP(ctx, launch_ksyrk(st, prec, m, Pc, kp, ns, sign, P0 > 0 ? 1 : 0, kpart));
+        }
+    }
+    PhaseScope ps(ctx, 4);
+    RJK_HIP(ctx, launch_kreduce(st, kpart, kp, ns, ctx->nbf, scale, vk_dev));
+    return RJK_OK;
+}
+
+}  // namespace
+
+extern "C" int rjk_vk_upper_occ_dev(rjk_ctx* ctx, const double* eigv_dev, size_t num_state, const size_t* homo,
+                                    const double* occ_dev, size_t spin_channel, double scaling_factor, int prec,
+                                    double* vk_dev) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!eigv_dev || !homo || !occ_dev || !vk_dev) return fail(ctx, RJK_ERR_ARG, "vk: NULL pointer");
+    const void* b = nullptr;
+    int rc = check_common(ctx, spin_channel, prec, &b);
+    if (rc) return rc;
+    if (prec != RJK_PREC_FP64) return fail(ctx, RJK_ERR_ARG, "vk: only RJK_PREC_FP64 is available in this build");
+    const int ns = (int)spin_channel;
+    size_t nw[2] = {0, 0};
+    for (int s = 0; s < ns; ++s) {
+        nw[s] = homo[s] + 1;
+        if (homo[s] >= num_state)
+            return fail(ctx, RJK_ERR_ARG, "vk: homo[" + std::to_string(s) + "] >= num_state");
+    }
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    KPlan kp;
+    k_plan_cols(kp, ctx->nbf, nw, ns);
+    double *cocc, *sign, *kpart, *m;
+    rc = k_plan_ws(ctx, kp, ns, &cocc, &sign, &kpart, &m);
+    if (rc) return rc;
+    {
+        PhaseScope ps(ctx, 4);
+        RJK_HIP(ctx, launch_prep_cocc(ctx->stream, eigv_dev, ctx->nbf, num_state, nw, occ_dev, ns, kp, cocc, sign));
+    }
+    return k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, vk_dev);
+}
+
+extern "C" int rjk_vk_upper_with_ri_v_sync(rjk_ctx* ctx, const double* eigv, size_t num_state, const size_t* homo,
+                                           const double* occupation, size_t spin_channel, double scaling_factor,
+                                           int prec, double* vk_upper) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!eigv || !homo || !occupation || !vk_upper) return fail(ctx, RJK_ERR_ARG, "vk: NULL host pointer");
+    if (spin_channel != 1 && spin_channel != 2)
+        return fail(ctx, RJK_ERR_ARG, "spin_channel must be 1 or 2, got " + std::to_string(spin_channel));
+    const size_t ne = spin_channel * ctx->nbf * num_state, no = spin_channel * num_state;
+    int rc = check_host_finite(ctx, eigv, ne, "eigv");
+    if (rc) return rc;
+    rc = check_host_finite(ctx, occupation, no, "occupation");
+    if (rc) return rc;
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf io;
+    const size_t out_n = spin_channel * ctx->npair;
+    RJK_HIP(ctx, hipMalloc(&io.p, (ne + no + out_n) * sizeof(double)));
+    double* d_e = (double*)io.p;
+    double* d_o = d_e + ne;
+    double* d_out = d_o + no;
+    RJK_HIP(ctx, hipMemcpyAsync(d_e, eigv, ne * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    RJK_HIP(ctx, hipMemcpyAsync(d_o, occupation, no * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    rc = rjk_vk_upper_occ_dev(ctx, d_e, num_state, homo, d_o, spin_channel, scaling_factor, prec, d_out);
+    if (rc) return rc;
+    RJK_HIP(ctx, hipMemcpyAsync(vk_upper, d_out, out_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RJK_OK;
+}
+
+namespace {
+
+int solver_handle(rjk_ctx* ctx, rocblas_handle* h) {
+    if (!ctx->solver) {
+        rocblas_handle hh;
+        if (rocblas_create_handle(&hh) != rocblas_status_success)
+            return fail(ctx, RJK_ERR_NOSOLVER, "rocblas_create_handle failed");
+        ctx->solver = hh;
+    }
+    rocblas_set_stream((rocblas_handle)ctx->solver, ctx->stream);
+    *h = (rocblas_handle)ctx->solver;
+    return RJK_OK;
+}
+
+int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scaling_factor, int prec,
+               double* vk_upper) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!dm || !vk_upper) return fail(ctx, RJK_ERR_ARG, "vk: NULL host pointer");
+    const void* b = nullptr;
+    int rc = check_common(ctx, spin_channel, prec, &b);
+    if (rc) return rc;
+    if (prec != RJK_PREC_FP64) return fail(ctx, RJK_ERR_ARG, "vk: only RJK_PREC_FP64 is available in this build");
+    const size_t nbf = ctx->nbf, n2 = nbf * nbf;
+    const int ns = (int)spin_channel;
+    rc = check_host_finite(ctx, dm, spin_channel * n2, "dm");
+    if (rc) return rc;
+    for (int s = 0; s < ns; ++s) {
+        const double* d = dm + s * n2;
+        double amax = 0, asym = 0;
+        for (size_t j = 0; j < nbf; ++j)
+            for (size_t i = 0; i < nbf; ++i) {
+                amax = std::max(amax, std::fabs(d[i + j * nbf]));
+                asym = std::max(asym, std::fabs(d[i + j * nbf] - d[j + i * nbf]));
+            }
+        if (asym > kAsymTol * std::max(amax, 1e-300))
+            return fail(ctx, RJK_ERR_ARG, "vk (density-only): dm is not symmetric (max|D-D^T| = " +
+                                              std::to_string(asym) + ")");
+    }
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    rocblas_handle h;
+    rc = solver_handle(ctx, &h);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    // eigen-decompose every spin's D (device), select the non-negligible eigenpairs
+    DevBuf io;
+    const size_t out_n = spin_channel * ctx->npair;
+    size_t io_n = spin_channel * n2 /*dm*/ + spin_channel * n2 /*vecs*/ + spin_channel * nbf /*lam*/ + nbf /*E*/ +
+                  spin_channel * nbf /*sel lam*/ + out_n;
+    RJK_HIP(ctx, hipMalloc(&io.p, io_n * sizeof(double) + 2 * spin_channel * nbf * sizeof(int) + 256));
+    double* d_dm = (double*)io.p;
+    double* d_vec = d_dm + spin_channel * n2;
+    double* d_lam = d_vec + spin_channel * n2;
+    double* d_E = d_lam + spin_channel * nbf;
+    double* d_sel = d_E + nbf;
+    double* d_out = d_sel + spin_channel * nbf;
+    int* d_idx = (int*)(d_out + out_n);
+    int* d_info = d_idx + spin_channel * nbf;
+    RJK_HIP(ctx, hipMemcpyAsync(d_dm, dm, spin_channel * n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    std::vector<double> lam(nbf);
+    std::vector<int> sel_idx[2];
+    std::vector<double> sel_lam[2];
+    size_t ncol[2] = {0, 0};
+    for (int s = 0; s < ns; ++s) {
+        RJK_HIP(ctx, launch_symmetrize(st, d_dm + s * n2, nbf, d_vec + s * n2));
+        rocblas_status rs = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)nbf,
+                                             d_vec + s * n2, (rocblas_int)nbf, d_lam + s * nbf, d_E, d_info);
+        if (rs != rocblas_status_success) return fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsyevd failed");
+        int info = 0;
+        RJK_HIP(ctx, hipMemcpyAsync(lam.data(), d_lam + s * nbf, nbf * sizeof(double), hipMemcpyDeviceToHost, st));
+        RJK_HIP(ctx, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+        RJK_HIP(ctx, hipStreamSynchronize(st));
+        if (info != 0) return fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsyevd info = " + std::to_string(info));
+        double lmax = 0;
+        for (double v : lam) lmax = std::max(lmax, std::fabs(v));
+        for (size_t q = 0; q < nbf; ++q)
+            if (std::fabs(lam[q]) > kEigRelCut * lmax) {
+                sel_idx[s].push_back((int)q);
+                sel_lam[s].push_back(lam[q]);
+            }
+        ncol[s] = sel_idx[s].size();
+    }
+    KPlan kp;
+    k_plan_cols(kp, nbf, ncol, ns);
+    double *cocc, *sign, *kpart, *m;
+    rc = k_plan_ws(ctx, kp, ns, &cocc, &sign, &kpart, &m);
+    if (rc) return rc;
+    RJK_HIP(ctx, hipMemsetAsync(cocc, 0, kp.nbf_pad * kp.nk * sizeof(double), st));
+    RJK_HIP(ctx, hipMemsetAsync(sign, 0, kp.nk * sizeof(double), st));
+    for (int s = 0; s < ns; ++s) {
+        if (!ncol[s]) continue;
+        RJK_HIP(ctx, hipMemcpyAsync(d_idx + s * nbf, sel_idx[s].data(), ncol[s] * sizeof(int), hipMemcpyHostToDevice,
+                                    st));
+        RJK_HIP(ctx, hipMemcpyAsync(d_sel + s * nbf, sel_lam[s].data(), ncol[s] * sizeof(double),
+                                    hipMemcpyHostToDevice, st));
+        RJK_HIP(ctx, launch_prep_cocc_idx(st, d_vec + s * n2, nbf, d_idx + s * nbf, d_sel + s * nbf, ncol[s],
+                                          kp.col_off[s], kp, cocc, sign));
+    }
+    rc = k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, d_out);
+    if (rc) return rc;
+    RJK_HIP(ctx, hipMemcpyAsync(vk_upper, d_out, out_n * sizeof(double), hipMemcpyDeviceToHost, st));
+    RJK_HIP(ctx, hipStreamSynchronize(st));
+    return RJK_OK;
+}
+
+}  // namespace
+
+extern "C" int rjk_vk_upper_with_rimatr_use_dm_only_sync(rjk_ctx* ctx, const double* dm, size_t spin_channel,
+                                                         double scaling_factor, int prec, double* vk_upper) {
+    return vk_dm_host(ctx, dm, spin_channel, scaling_factor, prec, vk_upper);
+}
+
+extern "C" int rjk_vk_upper_with_ri_v_use_dm_only_sync(rjk_ctx* ctx, const double* dm, size_t spin_channel,
+                                                       double scaling_factor, int prec, double* vk_upper) {
+    return vk_dm_host(ctx, dm, spin_channel, scaling_factor, prec, vk_upper);
+}
+
+// ---------------------------------------------------------------------------
+// measurement
+// ---------------------------------------------------------------------------
+extern "C" int rjk_profile_enable(rjk_ctx* ctx, int on) {
+    if (!ctx) return RJK_ERR_ARG;
+    ctx->prof = on != 0;
+    return RJK_OK;
+}
+
+extern "C" int rjk_profile_read(rjk_ctx* ctx, double* ms, uint64_t* launches) {
+    if (!ctx) return RJK_ERR_ARG;
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& e : ctx->ev_pending) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) {
+            ctx->prof_ms[e.phase] += t;
+            ctx->prof_n[e.phase] += 1;
+        }
+        ctx->ev_free.push_back(e);
+    }
+    ctx->ev_pending.clear();
+    for (int i = 0; i < kNumPhases; ++i) {
+        if (ms) ms[i] = ctx->prof_ms[i];
+        if (launches) launches[i] = ctx->prof_n[i];
+        ctx->prof_ms[i] = 0;
+        ctx->prof_n[i] = 0;
+    }
+    return RJK_OK;
+}
+
+extern "C" int rjk_copy_dev(rjk_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes) {
+    if (!ctx || !dst_dev || !src_dev) return ctx ? fail(ctx, RJK_ERR_ARG, "copy: NULL pointer") : RJK_ERR_ARG;
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    RJK_HIP(ctx, launch_copy(ctx->stream, dst_dev, src_dev, bytes));
+    return RJK_OK;
+}

@@ -1,0 +1,120 @@
+// rjk_internal.hpp — context, workspace and launcher declarations shared by
+// the C-ABI layer (rjk_api.cpp) and the kernel files (rjk_j.hip, rjk_k.hip,
+// rjk_util.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rest_jk.h"
+
+namespace rjk {
+
+constexpr size_t kLdAlign = 64;      // elements; device B columns start 512-B aligned (FP64)
+constexpr int kNumPhases = 5;        // J-gamma, J-acc, K-half, K-syrk, K-other
+constexpr int kTileMN = 128;         // nbf padding for the K tiles
+constexpr int kTileK = 16;           // k-block of the K tiles
+constexpr int kColPad = 64;          // occupied-column padding of M (all spins)
+
+inline size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct Status {
+    int code = RJK_OK;
+    std::string msg;
+};
+
+}  // namespace rjk
+
+struct rjk_ctx {
+    int device = 0;
+    size_t nbf = 0, naux = 0, npair = 0, ldb = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    double* b64 = nullptr;    // [naux][ldb]
+    float* b32 = nullptr;     // [naux][ldb]
+    size_t ws_limit = size_t(4) << 30;
+    void* ws = nullptr;       // grow-only workspace
+    size_t ws_size = 0;
+    void* solver = nullptr;   // rocblas_handle, created on first density-only K
+    std::string err;
+    // phase timing
+    bool prof = false;
+    double prof_ms[rjk::kNumPhases] = {0};
+    uint64_t prof_n[rjk::kNumPhases] = {0};
+    struct EvPair { hipEvent_t a, b; int phase; };
+    std::vector<EvPair> ev_pending;
+    std::vector<EvPair> ev_free;
+};
+
+namespace rjk {
+
+// ---- error helpers ------------------------------------------------------
+int fail(rjk_ctx* ctx, int code, const std::string& msg);
+#define RJK_HIP(ctx, call)                                                        \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return ::rjk::fail((ctx), RJK_ERR_HIP,                               \
+                               std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// workspace: returns a device buffer of at least `bytes` (grow-only)
+int ws_get(rjk_ctx* ctx, size_t bytes, void** out);
+
+// phase timing around a launch
+struct PhaseScope {
+    rjk_ctx* ctx;
+    rjk_ctx::EvPair ev;
+    bool on;
+    PhaseScope(rjk_ctx* c, int phase);
+    ~PhaseScope();
+};
+
+// ---- launchers (return hipError_t of the launch) ------------------------
+// J
+hipError_t launch_dm_weights(hipStream_t st, const double* dm, size_t nbf, size_t npair, size_t ldb,
+                             int nspin, int full_dm, double* w);
+hipError_t launch_jgamma(hipStream_t st, int prec, const void* b, size_t ldb, size_t naux, int nspin,
+                         const double* w, size_t nchunk, double* gpart);
+hipError_t launch_gamma_reduce(hipStream_t st, const double* gpart, size_t nchunk, int nspin, size_t naux,
+                               double* gamma);
+hipError_t launch_jacc(hipStream_t st, int prec, const void* b, size_t ldb, size_t naux, int nspin,
+                       const double* gamma, size_t nsplit, double* jpart);
+hipError_t launch_jreduce(hipStream_t st, const double* jpart, size_t nsplit, int nspin, size_t ldb,
+                          size_t npair, double scale, double* vj);
+void j_plan(size_t ldb, size_t naux, int prec, size_t* nchunk, size_t* nsplit);
+
+// K
+struct KPlan {
+    size_t nbf_pad = 0;      // multiple of kTileMN
+    size_t nk = 0;           // padded occupied columns, all spins (multiple of kColPad)
+    size_t col_off[2] = {0, 0}, col_cnt[2] = {0, 0};   // per-spin column ranges (multiples of 16)
+    size_t chunk = 0;        // P per chunk
+    size_t nsplit = 0;       // split-K slots of the SYRK
+};
+hipError_t launch_prep_cocc(hipStream_t st, const double* eigv, size_t nbf, size_t nstate,
+                            const size_t* nw, const double* occ, int nspin, const KPlan& kp,
+                            double* cocc, double* sign);
+hipError_t launch_prep_cocc_idx(hipStream_t st, const double* vecs, size_t nbf, const int* idx,
+                                const double* lam, size_t ncol, size_t col_off, const KPlan& kp,
+                                double* cocc, double* sign);
+hipError_t launch_khalf(hipStream_t st, int prec, const void* b, size_t ldb, size_t nbf, size_t P0,
+                        size_t Pc, const KPlan& kp, const void* cocc, void* m);
+hipError_t launch_ksyrk(hipStream_t st, int prec, const void* m, size_t Pc, const KPlan& kp, int nspin,
+                        const double* sign, int beta, double* kpart);
+hipError_t launch_kreduce(hipStream_t st, const double* kpart, const KPlan& kp, int nspin, size_t nbf,
+                          double scale, double* vk);
+
+// util
+hipError_t launch_fill_synth(hipStream_t st, uint64_t seed, size_t npair, size_t naux, size_t ldb, double* b);
+hipError_t launch_pack_ri3fn(hipStream_t st, const double* b3, size_t nbf, size_t nP, size_t ldb, double* b);
+hipError_t launch_pad_rimatr(hipStream_t st, const double* src, size_t npair, size_t nP, size_t ldb, double* b);
+hipError_t launch_to_fp32(hipStream_t st, const double* b, size_t n, float* out);
+hipError_t launch_symmetrize(hipStream_t st, const double* dm, size_t nbf, double* out);
+hipError_t launch_copy(hipStream_t st, void* dst, const void* src, size_t bytes);
+hipError_t launch_check_finite(hipStream_t st, const double* x, size_t n, int* flag);
+
+}  // namespace rjk

@@ -1,0 +1,314 @@
+// rjk_k.hip — exchange (K) build on packed B, occupied-orbital route.
+//
+//   K_s = scale · upper( Σ_P M_P diag(occ_s) M_Pᵀ ),  M_P = B_P C_s
+//
+// which is vk_upper_with_ri_v_sync (src/scf_io/mod.rs:2758-2834) evaluated on
+// the packed B of vk_upper_with_rimatr_use_dm_only_sync (:2703-2756).
+//
+// Columns: the occupied columns of all spins are concatenated (one B stream
+// serves both spins); column c of spin s holds C[:,c]·sqrt|occ_c| and a sign
+// sign_c = sgn(occ_c), so K = Σ_{P,c} sign_c M'_{P,c} M'_{P,c}ᵀ also for the
+// signed weights of the density-only route (D = U Λ Uᵀ).
+//
+// Stage 1 (K-half): M'[P][c][μ] = Σ_ν B_P[μ,ν] C'[ν][c] — a batched FP64-MFMA
+//   GEMM.  The A tile is read straight from the packed column and mirrored
+//   through the triangle while it is staged into LDS.
+// Stage 2 (K-SYRK): Kpart[split][s][ν][μ] (+)= Σ_{P,c} M'[P][c][μ] sign_c M'[P][c][ν]
+//   over upper tile pairs (μ-tile <= ν-tile), split-K over P so that the
+//   small nbf×nbf output still fills 256 CUs; deterministic (no atomics).
+// Stage 3 (K-reduce): vk_s[p(i,j)] = scale · Σ_split Kpart[split][s][j][i].
+//
+// MFMA: v_mfma_f64_16x16x4f64.  Lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; register r of lane l receives D[l/16 + 4r][l%16] (probed on
+// gfx950).  Both kernels put the memory-contiguous index (μ) on the D column
+// so that 16 consecutive lanes store 128 contiguous bytes.
+#include "rjk_internal.hpp"
+
+namespace rjk {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;           // 4 waves, 2×2
+constexpr int TK = kTileK;            // 16
+constexpr int kPadA = 4;              // LDS row padding (doubles)
+
+__device__ inline size_t tri(size_t j) { return j * (j + 1) / 2; }
+
+// ---------------------------------------------------------------------------
+// Stage 1: M'[P][c][μ] = Σ_ν Bsym_P[μ,ν] C'[ν][c]
+//   grid: (nbf_pad/TM, nk/TN, Pc)
+// ---------------------------------------------------------------------------
+template <int FM, int FN>
+__global__ __launch_bounds__(kBlock) void k_khalf_f64(const double* __restrict__ b, size_t ldb, int nbf, size_t P0,
+                                                      const double* __restrict__ cocc, int nk, int ldm,
+                                                      double* __restrict__ m) {
+    constexpr int TM = 2 * FM * 16, TN = 2 * FN * 16;
+    __shared__ double As[TK][TM + kPadA];
+    __shared__ double Bs[TK][TN + kPadA];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int mu0 = blockIdx.x * TM, c0 = blockIdx.y * TN;
+    const size_t Pl = blockIdx.z;
+    const double* __restrict__ bcol = b + (P0 + Pl) * ldb;
+
+    d4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (int nu0 = 0; nu0 < nbf; nu0 += TK) {
+        // ---- A tile: As[k][mm] = Bsym(mu0+mm, nu0+k)
+        const bool lower = (mu0 >= nu0 + TK - 1);  // every μ >= ν: contiguous along ν
+        if (lower) {
+#pragma unroll
+            for (int r = 0; r < TK * TM / kBlock; ++r) {
+                int e = tid + r * kBlock;
+                int k = e % TK, mm = e / TK;
+                int mu = mu0 + mm, nu = nu0 + k;
+                double v = 0.0;
+                if (mu < nbf && nu < nbf) v = bcol[tri((size_t)mu) + nu];
+                As[k][mm] = v;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < TK * TM / kBlock; ++r) {
+                int e = tid + r * kBlock;
+                int mm = e % TM, k = e / TM;
+                int mu = mu0 + mm, nu = nu0 + k;
+                double v = 0.0;
+                if (mu < nbf && nu < nbf) {
+                    int lo = mu < nu ? mu : nu, hi = mu < nu ? nu : mu;
+                    v = bcol[tri((size_t)hi) + lo];
+                }
+                As[k][mm] = v;
+            }
+        }
+        // ---- B tile: Bs[k][cc] = C'[nu0+k][c0+cc]   (C' rows padded with zeros)
+#pragma unroll
+        for (int r = 0; r < TK * TN / kBlock; ++r) {
+            int e = tid + r * kBlock;
+            int cc = e % TN, k = e / TN;
+            Bs[k][cc] = cocc[(size_t)(nu0 + k) * nk + c0 + cc];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 4) {
+            double bmu[FM], ac[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) bmu[i] = As[kk + (lane >> 4)][wm * FM * 16 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) ac[j] = Bs[kk + (lane >> 4)][wn * FN * 16 + j * 16 + (lane & 15)];
+            // D[c][μ] = Σ_ν C'[ν][c] Bsym[ν][μ]
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[j], bmu[i], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    double* __restrict__ mP = m + Pl * (size_t)nk * ldm;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int mu = mu0 + wm * FM * 16 + i * 16 + (lane & 15);
+            const int c = c0 + wn * FN * 16 + j * 16 + (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) mP[(size_t)(c + 4 * r) * ldm + mu] = acc[i][j][r];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2: Kpart (+)= Σ_{P,c} M'[P][c][μ] sign_c M'[P][c][ν] over the upper tiles.
+//   grid: (ntile(ntile+1)/2, nsplit, nspin)
+// ---------------------------------------------------------------------------
+struct SyrkCols {
+    int off[2], cnt[2];
+};
+
+template <int F>
+__global__ __launch_bounds__(kBlock) void k_ksyrk_f64(const double* __restrict__ m, int ldm, int nk, int Pc,
+                                                      SyrkCols cols, const double* __restrict__ sign,
+                                                      int p_per_split, int beta, double* __restrict__ kpart,
+                                                      size_t slot) {
+    constexpr int T = 2 * F * 16;
+    __shared__ double As[TK][T + kPadA];
+    __shared__ double Bs[TK][T + kPadA];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    // upper tile pair (tm <= tn) from the packed tile index
+    const size_t t = blockIdx.x;
+    size_t tn = (size_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (tri(tn) > t) --tn;
+    while (tri(tn + 1) <= t) ++tn;
+    const size_t tm = t - tri(tn);
+    const int mu0 = (int)tm * T, nu0 = (int)tn * T;
+    const int split = blockIdx.y, spin = blockIdx.z, nspin = gridDim.z;
+    const int cb = cols.off[spin], ce = cb + cols.cnt[spin];
+    const int Pb = split * p_per_split, Pe = min(Pc, Pb + p_per_split);
+
+    d4 acc[F][F];
+#pragma unroll
+    for (int i = 0; i < F; ++i)
+#pragma unroll
+        for (int j = 0; j < F; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (int P = Pb; P < Pe; ++P) {
+        for (int c = cb; c < ce; c += TK) {
+            const double* __restrict__ base = m + ((size_t)P * nk + c) * ldm;
+#pragma unroll
+            for (int r = 0; r < TK * T / kBlock; ++r) {
+                int e = tid + r * kBlock;
+                int mm = e % T, k = e / T;
+                As[k][mm] = base[(size_t)k * ldm + mu0 + mm];
+                Bs[k][mm] = sign[c + k] * base[(size_t)k * ldm + nu0 + mm];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < TK; kk += 4) {
+                double bmu[F], anu[F];
+#pragma unroll
+                for (int i = 0; i < F; ++i) bmu[i] = As[kk + (lane >> 4)][wm * F * 16 + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < F; ++j) anu[j] = Bs[kk + (lane >> 4)][wn * F * 16 + j * 16 + (lane & 15)];
+                // D[ν][μ] = Σ_k sign_k M'[k][ν] M'[k][μ]
+#pragma unroll
+                for (int i = 0; i < F; ++i)
+#pragma unroll
+                    for (int j = 0; j < F; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(anu[j], bmu[i], acc[i][j], 0, 0, 0);
+            }
+            __syncthreads();
+        }
+    }
+
+    double* __restrict__ kp = kpart + ((size_t)split * nspin + spin) * slot;
+#pragma unroll
+    for (int i = 0; i < F; ++i)
+#pragma unroll
+        for (int j = 0; j < F; ++j) {
+            const int mu = mu0 + wm * F * 16 + i * 16 + (lane & 15);
+            const int nu = nu0 + wn * F * 16 + j * 16 + (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double* dst = kp + (size_t)(nu + 4 * r) * ldm + mu;
+                double v = acc[i][j][r];
+                if (beta) v += *dst;
+                *dst = v;
+            }
+        }
+}
+
+__global__ void k_kreduce(const double* __restrict__ kpart, int nsplit, int nspin, size_t slot, int ldm,
+                          size_t npair, double scale, double* __restrict__ vk) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npair) return;
+    size_t j = (size_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while (tri(j) > p) --j;
+    while (tri(j + 1) <= p) ++j;
+    size_t i = p - tri(j);
+    for (int s = 0; s < nspin; ++s) {
+        double acc = 0;
+        for (int q = 0; q < nsplit; ++q) acc += kpart[((size_t)q * nspin + s) * slot + j * ldm + i];
+        vk[(size_t)s * npair + p] = acc * scale;
+    }
+}
+
+// C'[ν][c] and sign[c] from the orbitals: column c of spin s is orbital
+// (c - off_s) scaled by sqrt|occ|; padding columns and rows are zero.
+__global__ void k_prep_cocc(const double* __restrict__ eigv, int nbf, int nstate, int nw0, int nw1,
+                            const double* __restrict__ occ, int nspin, int off0, int off1, int nk, int nrow,
+                            double* __restrict__ cocc, double* __restrict__ sign) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nrow * nk) return;
+    int c = (int)(t % nk), nu = (int)(t / nk);
+    double v = 0.0, sg = 0.0;
+    int s = -1, o = 0;
+    if (c >= off0 && c < off0 + nw0) { s = 0; o = c - off0; }
+    else if (nspin == 2 && c >= off1 && c < off1 + nw1) { s = 1; o = c - off1; }
+    if (s >= 0) {
+        double w = occ[(size_t)s * nstate + o];
+        sg = (w > 0) ? 1.0 : ((w < 0) ? -1.0 : 0.0);
+        if (nu < nbf) v = eigv[(size_t)s * nbf * nstate + (size_t)o * nbf + nu] * sqrt(fabs(w));
+    }
+    cocc[t] = v;
+    if (nu == 0) sign[c] = sg;
+}
+
+// Columns gathered by index (density-only route): column col_off+q is
+// vecs[:, idx[q]] · sqrt|lam[q]|.
+__global__ void k_prep_cocc_idx(const double* __restrict__ vecs, int nbf, const int* __restrict__ idx,
+                                const double* __restrict__ lam, int ncol, int col_off, int nk, int nrow,
+                                double* __restrict__ cocc, double* __restrict__ sign) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nrow * ncol) return;
+    int q = (int)(t % ncol), nu = (int)(t / ncol);
+    double w = lam[q];
+    double v = (nu < nbf) ? vecs[(size_t)idx[q] * nbf + nu] * sqrt(fabs(w)) : 0.0;
+    cocc[(size_t)nu * nk + col_off + q] = v;
+    if (nu == 0) sign[col_off + q] = (w > 0) ? 1.0 : ((w < 0) ? -1.0 : 0.0);
+}
+
+}  // namespace
+
+constexpr int kHalfFM = 4, kHalfFN = 2;   // 128 (μ) × 64 (c) tiles
+constexpr int kSyrkF = 4;                 // 128 × 128 tiles
+
+hipError_t launch_prep_cocc(hipStream_t st, const double* eigv, size_t nbf, size_t nstate, const size_t* nw,
+                            const double* occ, int nspin, const KPlan& kp, double* cocc, double* sign) {
+    size_t n = kp.nbf_pad * kp.nk;
+    hipLaunchKernelGGL(k_prep_cocc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, eigv, (int)nbf,
+                       (int)nstate, (int)nw[0], nspin == 2 ? (int)nw[1] : 0, occ, nspin, (int)kp.col_off[0],
+                       (int)kp.col_off[1], (int)kp.nk, (int)kp.nbf_pad, cocc, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_cocc_idx(hipStream_t st, const double* vecs, size_t nbf, const int* idx, const double* lam,
+                                size_t ncol, size_t col_off, const KPlan& kp, double* cocc, double* sign) {
+    if (ncol == 0) return hipSuccess;
+    size_t n = kp.nbf_pad * ncol;
+    hipLaunchKernelGGL(k_prep_cocc_idx, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vecs, (int)nbf, idx,
+                       lam, (int)ncol, (int)col_off, (int)kp.nk, (int)kp.nbf_pad, cocc, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_khalf(hipStream_t st, int prec, const void* b, size_t ldb, size_t nbf, size_t P0, size_t Pc,
+                        const KPlan& kp, const void* cocc, void* m) {
+    (void)prec;
+    constexpr int TM = 2 * kHalfFM * 16, TN = 2 * kHalfFN * 16;
+    dim3 grid((unsigned)(kp.nbf_pad / TM), (unsigned)(kp.nk / TN), (unsigned)Pc);
+    hipLaunchKernelGGL((k_khalf_f64<kHalfFM, kHalfFN>), grid, dim3(kBlock), 0, st, (const double*)b, ldb, (int)nbf,
+                       P0, (const double*)cocc, (int)kp.nk, (int)kp.nbf_pad, (double*)m);
+    return hipGetLastError();
+}
+
+hipError_t launch_ksyrk(hipStream_t st, int prec, const void* m, size_t Pc, const KPlan& kp, int nspin,
+                        const double* sign, int beta, double* kpart) {
+    (void)prec;
+    constexpr int T = 2 * kSyrkF * 16;
+    size_t nt = kp.nbf_pad / T;
+    SyrkCols cols;
+    cols.off[0] = (int)kp.col_off[0];
+    cols.cnt[0] = (int)kp.col_cnt[0];
+    cols.off[1] = (int)kp.col_off[1];
+    cols.cnt[1] = (int)kp.col_cnt[1];
+    int pps = (int)((Pc + kp.nsplit - 1) / kp.nsplit);
+    dim3 grid((unsigned)(nt * (nt + 1) / 2), (unsigned)kp.nsplit, (unsigned)nspin);
+    hipLaunchKernelGGL((k_ksyrk_f64<kSyrkF>), grid, dim3(kBlock), 0, st, (const double*)m, (int)kp.nbf_pad,
+                       (int)kp.nk, (int)Pc, cols, sign, pps, beta, kpart, kp.nbf_pad * kp.nbf_pad);
+    return hipGetLastError();
+}
+
+hipError_t launch_kreduce(hipStream_t st, const double* kpart, const KPlan& kp, int nspin, size_t nbf, double scale,
+                          double* vk) {
+    size_t npair = nbf * (nbf + 1) / 2;
+    hipLaunchKernelGGL(k_kreduce, dim3((unsigned)((npair + 255) / 256)), dim3(256), 0, st, kpart, (int)kp.nsplit,
+                       nspin, kp.nbf_pad * kp.nbf_pad, (int)kp.nbf_pad, npair, scale, vk);
+    return hipGetLastError();
+}
+
+}  // namespace rjk

@@ -1,0 +1,324 @@
+"""Python host binding of the MI355X RI-V J/K library (rest_amd/lib/librest_jk.so).
+
+The module-level functions mirror REST's vj/vk builders (src/scf_io/mod.rs)
+with the same names, argument meaning and return shape:
+
+    vj_upper_with_rimatr_v_sync(ri3fn, dm, spin_channel, scaling_factor)        # :2490
+    vj_upper_with_ri_v_sync(ri3fn, dm, spin_channel, scaling_factor)            # :2428
+    vk_upper_with_ri_v_sync(ri3fn, eigv, homo, occupation, spin_channel,
+                            scaling_factor)                                     # :2758
+    vk_upper_with_rimatr_use_dm_only_sync(ri3fn, dm, spin_channel, scaling_factor)  # :2703
+    vk_upper_with_ri_v_use_dm_only_sync(ri3fn, dm, spin_channel, scaling_factor)    # :2640
+
+`ri3fn` is a `DeviceRI` (B resident in HBM) or None; like REST, None yields
+two 1-element zero vectors.  Every builder returns a list of two packed-upper
+vectors (p = j(j+1)/2 + i, i <= j); when spin_channel == 1 the second is
+REST's 1-element dummy.
+
+Array conventions: matrices are numpy 2-D arrays indexed [row, col] (they are
+handed to the library column-major, like REST's MatrixFull); packed B
+("rimatr") is indexed [p, P] with shape (npair, naux); full B ("ri3fn") is
+indexed [i, j, P] with shape (nbf, nbf, naux).
+
+There is no CPU fallback: if the native library is absent every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "librest_jk.so"
+
+RJK_OK = 0
+PREC_FP64, PREC_FP32, PREC_BF16X2 = 0, 1, 2
+STORE_FP64, STORE_FP32, STORE_BF16X2 = 1, 2, 4
+ERRORS = {1: "ARG", 2: "HIP", 3: "NOMEM", 4: "STATE", 5: "NONFINITE", 6: "NOSOLVER"}
+
+# every symbol include/rest_jk.h declares (checked by tests/test_abi.py)
+EXPORTS = [
+    "rjk_create", "rjk_destroy", "rjk_last_error", "rjk_num_basis", "rjk_num_auxbas", "rjk_num_baspar",
+    "rjk_ld_baspar", "rjk_set_stream", "rjk_synchronize", "rjk_set_workspace_limit", "rjk_set_rimatr",
+    "rjk_set_rimatr_dev", "rjk_set_ri3fn", "rjk_fill_rimatr_synthetic", "rjk_rimatr_dev",
+    "rjk_vj_upper_with_rimatr_v_sync", "rjk_vj_upper_with_ri_v_sync", "rjk_vj_upper_dev",
+    "rjk_vk_upper_with_ri_v_sync", "rjk_vk_upper_occ_dev", "rjk_vk_upper_with_rimatr_use_dm_only_sync",
+    "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_profile_enable", "rjk_profile_read", "rjk_copy_dev",
+]
+
+
+class RJKError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rjk error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def load_library(path: Optional[Path] = None) -> C.CDLL:
+    """Load librest_jk.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"native J/K library not built: {p} (run `python -m rest_amd.build`)")
+    # PyTorch wheels bundle their own libamdhip64.so.7 / libhsa-runtime64.so.1.
+    # Loading torch first makes this library bind to those same copies (the
+    # loader matches by SONAME), so one process never holds two HIP runtimes.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(str(p))
+    vp, sz, d, pd, i = C.c_void_p, C.c_size_t, C.c_double, C.POINTER(C.c_double), C.c_int
+    sig = {
+        "rjk_create": (i, [i, sz, sz, C.POINTER(vp)]),
+        "rjk_destroy": (i, [vp]),
+        "rjk_last_error": (C.c_char_p, [vp]),
+        "rjk_num_basis": (sz, [vp]),
+        "rjk_num_auxbas": (sz, [vp]),
+        "rjk_num_baspar": (sz, [vp]),
+        "rjk_ld_baspar": (sz, [vp]),
+        "rjk_set_stream": (i, [vp, vp]),
+        "rjk_synchronize": (i, [vp]),
+        "rjk_set_workspace_limit": (i, [vp, sz]),
+        "rjk_set_rimatr": (i, [vp, vp, C.c_uint]),
+        "rjk_set_rimatr_dev": (i, [vp, vp, C.c_uint]),
+        "rjk_set_ri3fn": (i, [vp, vp, C.c_uint]),
+        "rjk_fill_rimatr_synthetic": (i, [vp, C.c_uint64, C.c_uint]),
+        "rjk_rimatr_dev": (vp, [vp]),
+        "rjk_vj_upper_with_rimatr_v_sync": (i, [vp, vp, sz, d, i, vp]),
+        "rjk_vj_upper_with_ri_v_sync": (i, [vp, vp, sz, d, i, vp]),
+        "rjk_vj_upper_dev": (i, [vp, vp, sz, d, i, i, vp]),
+        "rjk_vk_upper_with_ri_v_sync": (i, [vp, vp, sz, vp, vp, sz, d, i, vp]),
+        "rjk_vk_upper_occ_dev": (i, [vp, vp, sz, vp, vp, sz, d, i, vp]),
+        "rjk_vk_upper_with_rimatr_use_dm_only_sync": (i, [vp, vp, sz, d, i, vp]),
+        "rjk_vk_upper_with_ri_v_use_dm_only_sync": (i, [vp, vp, sz, d, i, vp]),
+        "rjk_profile_enable": (i, [vp, i]),
+        "rjk_profile_read": (i, [vp, pd, C.POINTER(C.c_uint64)]),
+        "rjk_copy_dev": (i, [vp, vp, vp, sz]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def _colmajor_stack(mats: Sequence[np.ndarray], spin_channel: int, shape) -> np.ndarray:
+    """Stack spin matrices as contiguous column-major blocks."""
+    out = np.empty((spin_channel,) + tuple(shape[::-1]), dtype=np.float64)
+    for s in range(spin_channel):
+        m = np.asarray(mats[s], dtype=np.float64)
+        if m.shape != tuple(shape):
+            raise ValueError(f"matrix of spin {s} has shape {m.shape}, expected {tuple(shape)}")
+        out[s] = m.T  # row-major of the transpose == column-major of m
+    return out
+
+
+class DeviceRI:
+    """B[μν,P] resident in HBM (packed pair layout), one context per device."""
+
+    PHASES = ("j_gamma", "j_acc", "k_half", "k_syrk", "other")
+
+    def __init__(self, num_basis: int, num_auxbas: int, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.rjk_create(int(device), int(num_basis), int(num_auxbas), C.byref(h))
+        if rc != RJK_OK:
+            raise RJKError(rc, self.lib.rjk_last_error(None).decode())
+        self.h = h
+        self.nbf = int(num_basis)
+        self.naux = int(num_auxbas)
+        self.npair = self.nbf * (self.nbf + 1) // 2
+        self.ldb = int(self.lib.rjk_ld_baspar(h))
+
+    # -- lifecycle -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rjk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != RJK_OK:
+            raise RJKError(rc, self.lib.rjk_last_error(self.h).decode())
+
+    # -- B storage -------------------------------------------------------------
+    def set_rimatr(self, rimatr: np.ndarray, store: int = STORE_FP64):
+        """rimatr indexed [p, P], shape (npair, naux)."""
+        b = np.asarray(rimatr, dtype=np.float64)
+        if b.shape != (self.npair, self.naux):
+            raise ValueError(f"rimatr shape {b.shape}, expected {(self.npair, self.naux)}")
+        buf = np.ascontiguousarray(b.T)  # [P][p] == column-major [p, P]
+        self._check(self.lib.rjk_set_rimatr(self.h, _ptr(buf), store))
+
+    def set_ri3fn(self, ri3fn: np.ndarray, store: int = STORE_FP64):
+        """ri3fn indexed [i, j, P], shape (nbf, nbf, naux)."""
+        b = np.asarray(ri3fn, dtype=np.float64)
+        if b.shape != (self.nbf, self.nbf, self.naux):
+            raise ValueError(f"ri3fn shape {b.shape}, expected {(self.nbf, self.nbf, self.naux)}")
+        buf = np.ascontiguousarray(np.transpose(b, (2, 1, 0)))  # [P][j][i]
+        self._check(self.lib.rjk_set_ri3fn(self.h, _ptr(buf), store))
+
+    def fill_synthetic(self, seed: int, store: int = STORE_FP64):
+        self._check(self.lib.rjk_fill_rimatr_synthetic(self.h, C.c_uint64(seed), store))
+
+    def set_rimatr_dev(self, ptr: int, store: int = STORE_FP64):
+        self._check(self.lib.rjk_set_rimatr_dev(self.h, C.c_void_p(ptr), store))
+
+    def rimatr_dev_ptr(self) -> int:
+        return int(self.lib.rjk_rimatr_dev(self.h) or 0)
+
+    def set_stream(self, stream_ptr: Optional[int]):
+        self._check(self.lib.rjk_set_stream(self.h, C.c_void_p(stream_ptr or 0)))
+
+    def synchronize(self):
+        self._check(self.lib.rjk_synchronize(self.h))
+
+    def set_workspace_limit(self, nbytes: int):
+        self._check(self.lib.rjk_set_workspace_limit(self.h, int(nbytes)))
+
+    # -- builders (host arrays) -----------------------------------------------
+    def vj(self, dm: Sequence[np.ndarray], spin_channel: int, scaling_factor: float = 1.0,
+           prec: int = PREC_FP64, full_dm: bool = False) -> np.ndarray:
+        d = _colmajor_stack(dm, spin_channel, (self.nbf, self.nbf))
+        out = np.empty((spin_channel, self.npair))
+        f = self.lib.rjk_vj_upper_with_ri_v_sync if full_dm else self.lib.rjk_vj_upper_with_rimatr_v_sync
+        self._check(f(self.h, _ptr(d), spin_channel, float(scaling_factor), prec, _ptr(out)))
+        return out
+
+    def vk_occ(self, eigv: Sequence[np.ndarray], homo: Sequence[int], occupation: Sequence[np.ndarray],
+               spin_channel: int, scaling_factor: float = 1.0, prec: int = PREC_FP64) -> np.ndarray:
+        nstate = np.asarray(eigv[0]).shape[1]
+        e = _colmajor_stack(eigv, spin_channel, (self.nbf, nstate))
+        occ = np.zeros((spin_channel, nstate))
+        for s in range(spin_channel):
+            occ[s, :] = np.asarray(occupation[s], dtype=np.float64)[:nstate]
+        hm = (C.c_size_t * 2)(*(list(homo)[:spin_channel] + [0] * (2 - spin_channel)))
+        out = np.empty((spin_channel, self.npair))
+        self._check(self.lib.rjk_vk_upper_with_ri_v_sync(self.h, _ptr(e), nstate, C.cast(hm, C.c_void_p),
+                                                         _ptr(occ), spin_channel, float(scaling_factor), prec,
+                                                         _ptr(out)))
+        return out
+
+    def vk_dm(self, dm: Sequence[np.ndarray], spin_channel: int, scaling_factor: float = 1.0,
+              prec: int = PREC_FP64) -> np.ndarray:
+        d = _colmajor_stack(dm, spin_channel, (self.nbf, self.nbf))
+        out = np.empty((spin_channel, self.npair))
+        self._check(self.lib.rjk_vk_upper_with_rimatr_use_dm_only_sync(self.h, _ptr(d), spin_channel,
+                                                                       float(scaling_factor), prec, _ptr(out)))
+        return out
+
+    # -- builders (device pointers, stream-ordered) -----------------------------
+    def vj_dev(self, dm_ptr: int, spin_channel: int, scaling_factor: float, out_ptr: int,
+               prec: int = PREC_FP64, full_dm: bool = False):
+        self._check(self.lib.rjk_vj_upper_dev(self.h, C.c_void_p(dm_ptr), spin_channel, float(scaling_factor),
+                                              prec, int(full_dm), C.c_void_p(out_ptr)))
+
+    def vk_occ_dev(self, eigv_ptr: int, num_state: int, homo: Sequence[int], occ_ptr: int, spin_channel: int,
+                   scaling_factor: float, out_ptr: int, prec: int = PREC_FP64):
+        hm = (C.c_size_t * 2)(*(list(homo)[:spin_channel] + [0] * (2 - spin_channel)))
+        self._check(self.lib.rjk_vk_upper_occ_dev(self.h, C.c_void_p(eigv_ptr), num_state, C.cast(hm, C.c_void_p),
+                                                  C.c_void_p(occ_ptr), spin_channel, float(scaling_factor), prec,
+                                                  C.c_void_p(out_ptr)))
+
+    # -- measurement -------------------------------------------------------------
+    def profile_enable(self, on: bool = True):
+        self._check(self.lib.rjk_profile_enable(self.h, int(on)))
+
+    def profile_read(self) -> dict:
+        ms = (C.c_double * 5)()
+        n = (C.c_uint64 * 5)()
+        self._check(self.lib.rjk_profile_read(self.h, ms, n))
+        return {name: (ms[k], int(n[k])) for k, name in enumerate(self.PHASES)}
+
+    def copy_dev(self, dst_ptr: int, src_ptr: int, nbytes: int):
+        self._check(self.lib.rjk_copy_dev(self.h, C.c_void_p(dst_ptr), C.c_void_p(src_ptr), int(nbytes)))
+
+
+# ---------------------------------------------------------------------------
+# REST-shaped builders (src/scf_io/mod.rs)
+# ---------------------------------------------------------------------------
+def _dummy_pair(spin_channel: int, vals: np.ndarray) -> list:
+    out = [np.zeros(1), np.zeros(1)]
+    for s in range(spin_channel):
+        out[s] = vals[s].copy()
+    return out
+
+
+def vj_upper_with_rimatr_v_sync(ri3fn: Optional[DeviceRI], dm, spin_channel: int, scaling_factor: float,
+                                prec: int = PREC_FP64) -> list:
+    """src/scf_io/mod.rs:2490 — packed B, upper triangle of D."""
+    if ri3fn is None:
+        return [np.zeros(1), np.zeros(1)]
+    return _dummy_pair(spin_channel, ri3fn.vj(dm, spin_channel, scaling_factor, prec, full_dm=False))
+
+
+def vj_upper_with_ri_v_sync(ri3fn: Optional[DeviceRI], dm, spin_channel: int, scaling_factor: float,
+                            prec: int = PREC_FP64) -> list:
+    """src/scf_io/mod.rs:2428 — full B, γ over the full D."""
+    if ri3fn is None:
+        return [np.zeros(1), np.zeros(1)]
+    return _dummy_pair(spin_channel, ri3fn.vj(dm, spin_channel, scaling_factor, prec, full_dm=True))
+
+
+def vk_upper_with_ri_v_sync(ri3fn: Optional[DeviceRI], eigv, homo, occupation, spin_channel: int,
+                            scaling_factor: float, prec: int = PREC_FP64) -> list:
+    """src/scf_io/mod.rs:2758 — occupied-orbital route."""
+    if ri3fn is None:
+        return [np.zeros(1), np.zeros(1)]
+    return _dummy_pair(spin_channel, ri3fn.vk_occ(eigv, homo, occupation, spin_channel, scaling_factor, prec))
+
+
+def vk_upper_with_rimatr_use_dm_only_sync(ri3fn: Optional[DeviceRI], dm, spin_channel: int, scaling_factor: float,
+                                          prec: int = PREC_FP64) -> list:
+    """src/scf_io/mod.rs:2703 — density-only route on packed B."""
+    if ri3fn is None:
+        return [np.zeros(1), np.zeros(1)]
+    return _dummy_pair(spin_channel, ri3fn.vk_dm(dm, spin_channel, scaling_factor, prec))
+
+
+def vk_upper_with_ri_v_use_dm_only_sync(ri3fn: Optional[DeviceRI], dm, spin_channel: int, scaling_factor: float,
+                                        prec: int = PREC_FP64) -> list:
+    """src/scf_io/mod.rs:2640 — density-only route on full B (stored through its upper triangle)."""
+    return vk_upper_with_rimatr_use_dm_only_sync(ri3fn, dm, spin_channel, scaling_factor, prec)
+
+
+# ---------------------------------------------------------------------------
+# packed-index helpers
+# ---------------------------------------------------------------------------
+def upper_indices(nbf: int):
+    """(i, j) arrays of the packed order p = j(j+1)/2 + i, i <= j."""
+    j = np.concatenate([np.full(k + 1, k) for k in range(nbf)])
+    i = np.concatenate([np.arange(k + 1) for k in range(nbf)])
+    return i, j
+
+
+def unpack_upper(v: np.ndarray, nbf: int) -> np.ndarray:
+    """packed upper vector -> symmetric full matrix"""
+    i, j = upper_indices(nbf)
+    m = np.zeros((nbf, nbf))
+    m[i, j] = v
+    m[j, i] = v
+    return m

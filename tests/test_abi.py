@@ -1,0 +1,67 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point that
+include/rest_jk.h declares, and its host-side argument checks work without a
+GPU (no compute call is made here)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from rest_amd import jk
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_symbols():
+    text = (ROOT / "include" / "rest_jk.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rjk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = jk.load_library()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(jk.EXPORTS) == syms
+
+
+def test_create_rejects_bad_shapes_before_touching_a_device():
+    lib = jk.load_library()
+    h = C.c_void_p()
+    assert lib.rjk_create(0, 0, 10, C.byref(h)) == 1
+    assert b"positive" in lib.rjk_last_error(None)
+    assert lib.rjk_create(0, 10, 0, C.byref(h)) == 1
+    assert lib.rjk_create(0, 10, 10, None) == 1
+    assert not h.value
+
+
+def test_create_without_device_reports_error():
+    lib = jk.load_library()
+    h = C.c_void_p()
+    rc = lib.rjk_create(999, 24, 113, C.byref(h))
+    assert rc == 2
+    assert b"not available" in lib.rjk_last_error(None)
+
+
+def test_null_context_calls_are_rejected():
+    lib = jk.load_library()
+    assert lib.rjk_destroy(None) == 0
+    assert lib.rjk_synchronize(None) == 1
+    assert lib.rjk_num_basis(None) == 0
+    assert lib.rjk_vj_upper_with_rimatr_v_sync(None, None, 1, 1.0, 0, None) == 1
+    assert lib.rjk_vk_upper_with_ri_v_sync(None, None, 1, None, None, 1, 1.0, 0, None) == 1
+
+
+def test_reference_shaped_builders_return_dummy_when_b_is_absent():
+    """REST returns two 1-element zero vectors when ri3fn is None (src/scf_io/mod.rs:2494)."""
+    out = jk.vj_upper_with_rimatr_v_sync(None, [], 1, 1.0)
+    assert len(out) == 2 and all(v.shape == (1,) and v[0] == 0 for v in out)
+    out = jk.vk_upper_with_ri_v_sync(None, [], [0, 0], [], 2, -1.0)
+    assert len(out) == 2
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(RuntimeError, match="not built"):
+        jk.load_library(tmp_path / "nope.so")
