@@ -108,7 +108,10 @@ def main():
     d = c @ c.T
     ri = jk.DeviceRI(nbf, naux)
     ri.fill_synthetic(seed)
-    stream = torch.cuda.current_stream(dev)
+    # a non-default torch stream (default priority) shared by the library and
+    # the timing events: the null stream would hand the library back its own
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     ri.set_stream(stream.cuda_stream)
 
     d_dev = torch.from_numpy(np.ascontiguousarray(d.T)).to(dev)            # column-major D

@@ -216,3 +216,14 @@ def cpu_baseline_vk(rimatr_cols: np.ndarray, nbf: int, c: np.ndarray, occ: np.nd
     out = np.empty(nbf * (nbf + 1) // 2)
     t = lib().orc_cpu_baseline_vk(_p(b), nbf, b.shape[0], _p(cc), nw, _p(o), _p(out), threads)
     return t, out
+
+
+def fock_upper(hcore_upper: np.ndarray, vj: np.ndarray, vk: np.ndarray, spin_channel: int) -> np.ndarray:
+    """F_s = H + J_0 (+ J_1) + K_s, all packed upper; vj/vk shaped (spin_channel, npair)."""
+    h = np.ascontiguousarray(hcore_upper, dtype=np.float64)
+    j = np.ascontiguousarray(vj, dtype=np.float64)
+    k = np.ascontiguousarray(vk, dtype=np.float64)
+    npair = h.shape[0]
+    out = np.empty((spin_channel, npair))
+    lib().orc_fock_upper(_p(h), _p(j), _p(k), npair, spin_channel, _p(out))
+    return out

@@ -24,6 +24,10 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-I", str(PKG.parent / "include")]
 SOURCES = ["rjk_api.cpp", "rjk_j.hip", "rjk_k.hip", "rjk_util.hip"]
+# MFMA accumulators in VGPRs: the K tiles carry 64-128 accumulator registers
+# across a software-pipelined loop; the default AGPR form makes the allocator
+# copy them VGPR<->AGPR every iteration.
+PER_FILE = {"rjk_k.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 
@@ -31,7 +35,7 @@ def _compile(src: Path) -> Path:
     obj = OBJ / (src.name + ".o")
     cmd = [HIPCC, *COMMON]
     if src.suffix == ".hip":
-        cmd += ["-x", "hip", *DEVICE_FLAGS]
+        cmd += ["-x", "hip", *DEVICE_FLAGS, *PER_FILE.get(src.name, [])]
     cmd += ["-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

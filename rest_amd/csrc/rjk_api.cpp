@@ -381,7 +381,8 @@ int k_plan_ws(rjk_ctx* ctx, KPlan& kp, int ns, double** cocc, double** sign, dou
     const size_t small = round_up(kp.nbf_pad * kp.nk * sizeof(double), 256) + round_up(kp.nk * sizeof(double), 256);
     // split-K slots: enough work-groups, at most a quarter of the workspace,
     // and at least 8 auxiliary columns per slot in every chunk
-    size_t nsplit = std::max<size_t>(1, (target + ntiles * ns - 1) / (ntiles * ns));
+    // whole rounds of work-groups: floor, so that the last round is nearly full
+    size_t nsplit = std::max<size_t>(1, target / (ntiles * ns));
     nsplit = std::min(nsplit, std::max<size_t>(1, (ctx->ws_limit / 4) / slot_bytes));
     size_t chunk = 1;
     for (int it = 0; it < 3; ++it) {

@@ -37,20 +37,78 @@ __device__ inline size_t tri(size_t j) { return j * (j + 1) / 2; }
 
 // ---------------------------------------------------------------------------
 // Stage 1: M'[P][c][μ] = Σ_ν Bsym_P[μ,ν] C'[ν][c]
-//   grid: (nbf_pad/TM, nk/TN, Pc)
+//   grid: (nbf_pad/TM, nk/TN, Pc).  Software pipeline: the next k-block is
+//   fetched into registers while the MFMAs consume the current one from LDS;
+//   LDS is double-buffered, one barrier per k-block.
 // ---------------------------------------------------------------------------
 template <int FM, int FN>
 __global__ __launch_bounds__(kBlock) void k_khalf_f64(const double* __restrict__ b, size_t ldb, int nbf, size_t P0,
                                                       const double* __restrict__ cocc, int nk, int ldm,
                                                       double* __restrict__ m) {
     constexpr int TM = 2 * FM * 16, TN = 2 * FN * 16;
-    __shared__ double As[TK][TM + kPadA];
-    __shared__ double Bs[TK][TN + kPadA];
+    constexpr int NA = TK * TM / kBlock, NB = TK * TN / kBlock;
+    __shared__ double As[2][TK][TM + kPadA];
+    __shared__ double Bs[2][TK][TN + kPadA];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int mu0 = blockIdx.x * TM, c0 = blockIdx.y * TN;
     const size_t Pl = blockIdx.z;
     const double* __restrict__ bcol = b + (P0 + Pl) * ldb;
+    double ra[NA], rb[NB];
+    unsigned live = 0;   // bit r: element r of the A fetch lies inside nbf×nbf
+
+    // A tile As[k][mm] = Bsym(mu0+mm, nu0+k).  Tiles wholly below the diagonal
+    // are read along ν (contiguous there), all others along μ.  Out-of-range
+    // elements read a valid address and are zeroed when stashed, so nothing
+    // waits on the loads before the MFMAs of the current block.
+    auto fetch = [&](int nu0) {
+        live = 0;
+        if (mu0 >= nu0 + TK - 1) {
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+                const int e = tid + r * kBlock, k = e % TK, mm = e / TK;
+                const int mu = mu0 + mm, nu = nu0 + k;
+                const bool ok = mu < nbf && nu < nbf;
+                live |= (unsigned)ok << r;
+                ra[r] = bcol[ok ? tri((size_t)mu) + nu : 0];
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+                const int e = tid + r * kBlock, mm = e % TM, k = e / TM;
+                const int mu = mu0 + mm, nu = nu0 + k;
+                const int lo = mu < nu ? mu : nu, hi = mu < nu ? nu : mu;
+                const bool ok = mu < nbf && nu < nbf;
+                live |= (unsigned)ok << r;
+                ra[r] = bcol[ok ? tri((size_t)hi) + lo : 0];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + r * kBlock, cc = e % TN, k = e / TN;
+            rb[r] = cocc[(size_t)(nu0 + k) * nk + c0 + cc];   // C' rows are zero-padded to nbf_pad
+        }
+    };
+    auto stash = [&](int buf, int nu0) {
+        if (mu0 >= nu0 + TK - 1) {
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+                const int e = tid + r * kBlock;
+                As[buf][e % TK][e / TK] = ((live >> r) & 1u) ? ra[r] : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < NA; ++r) {
+                const int e = tid + r * kBlock;
+                As[buf][e / TM][e % TM] = ((live >> r) & 1u) ? ra[r] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + r * kBlock;
+            Bs[buf][e / TN][e % TN] = rb[r];
+        }
+    };
 
     d4 acc[FM][FN];
 #pragma unroll
@@ -58,48 +116,21 @@ __global__ __launch_bounds__(kBlock) void k_khalf_f64(const double* __restrict__
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    for (int nu0 = 0; nu0 < nbf; nu0 += TK) {
-        // ---- A tile: As[k][mm] = Bsym(mu0+mm, nu0+k)
-        const bool lower = (mu0 >= nu0 + TK - 1);  // every μ >= ν: contiguous along ν
-        if (lower) {
-#pragma unroll
-            for (int r = 0; r < TK * TM / kBlock; ++r) {
-                int e = tid + r * kBlock;
-                int k = e % TK, mm = e / TK;
-                int mu = mu0 + mm, nu = nu0 + k;
-                double v = 0.0;
-                if (mu < nbf && nu < nbf) v = bcol[tri((size_t)mu) + nu];
-                As[k][mm] = v;
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < TK * TM / kBlock; ++r) {
-                int e = tid + r * kBlock;
-                int mm = e % TM, k = e / TM;
-                int mu = mu0 + mm, nu = nu0 + k;
-                double v = 0.0;
-                if (mu < nbf && nu < nbf) {
-                    int lo = mu < nu ? mu : nu, hi = mu < nu ? nu : mu;
-                    v = bcol[tri((size_t)hi) + lo];
-                }
-                As[k][mm] = v;
-            }
-        }
-        // ---- B tile: Bs[k][cc] = C'[nu0+k][c0+cc]   (C' rows padded with zeros)
-#pragma unroll
-        for (int r = 0; r < TK * TN / kBlock; ++r) {
-            int e = tid + r * kBlock;
-            int cc = e % TN, k = e / TN;
-            Bs[k][cc] = cocc[(size_t)(nu0 + k) * nk + c0 + cc];
-        }
-        __syncthreads();
+    const int nkb = (nbf + TK - 1) / TK;
+    fetch(0);
+    stash(0, 0);
+    __syncthreads();
+    for (int kb = 0; kb < nkb; ++kb) {
+        const int cur = kb & 1;
+        const bool more = kb + 1 < nkb;
+        if (more) fetch((kb + 1) * TK);
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 4) {
             double bmu[FM], ac[FN];
 #pragma unroll
-            for (int i = 0; i < FM; ++i) bmu[i] = As[kk + (lane >> 4)][wm * FM * 16 + i * 16 + (lane & 15)];
+            for (int i = 0; i < FM; ++i) bmu[i] = As[cur][kk + (lane >> 4)][wm * FM * 16 + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) ac[j] = Bs[kk + (lane >> 4)][wn * FN * 16 + j * 16 + (lane & 15)];
+            for (int j = 0; j < FN; ++j) ac[j] = Bs[cur][kk + (lane >> 4)][wn * FN * 16 + j * 16 + (lane & 15)];
             // D[c][μ] = Σ_ν C'[ν][c] Bsym[ν][μ]
 #pragma unroll
             for (int i = 0; i < FM; ++i)
@@ -107,6 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_khalf_f64(const double* __restrict__
                 for (int j = 0; j < FN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[j], bmu[i], acc[i][j], 0, 0, 0);
         }
+        if (more) stash(cur ^ 1, (kb + 1) * TK);
         __syncthreads();
     }
 
@@ -124,7 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_khalf_f64(const double* __restrict__
 
 // ---------------------------------------------------------------------------
 // Stage 2: Kpart (+)= Σ_{P,c} M'[P][c][μ] sign_c M'[P][c][ν] over the upper tiles.
-//   grid: (ntile(ntile+1)/2, nsplit, nspin)
+//   grid: (ntile(ntile+1)/2, nsplit, nspin); same pipeline as stage 1 over the
+//   flattened (P, 16-column block) reduction index.
 // ---------------------------------------------------------------------------
 struct SyrkCols {
     int off[2], cnt[2];
@@ -136,8 +169,9 @@ __global__ __launch_bounds__(kBlock) void k_ksyrk_f64(const double* __restrict__
                                                       int p_per_split, int beta, double* __restrict__ kpart,
                                                       size_t slot) {
     constexpr int T = 2 * F * 16;
-    __shared__ double As[TK][T + kPadA];
-    __shared__ double Bs[TK][T + kPadA];
+    constexpr int NL = TK * T / kBlock;
+    __shared__ double As[2][TK][T + kPadA];
+    __shared__ double Bs[2][TK][T + kPadA];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     // upper tile pair (tm <= tn) from the packed tile index
@@ -148,8 +182,35 @@ __global__ __launch_bounds__(kBlock) void k_ksyrk_f64(const double* __restrict__
     const size_t tm = t - tri(tn);
     const int mu0 = (int)tm * T, nu0 = (int)tn * T;
     const int split = blockIdx.y, spin = blockIdx.z, nspin = gridDim.z;
-    const int cb = cols.off[spin], ce = cb + cols.cnt[spin];
+    const int cb = cols.off[spin];
+    const int ncb = cols.cnt[spin] / TK;
     const int Pb = split * p_per_split, Pe = min(Pc, Pb + p_per_split);
+    const int nq = (Pe > Pb ? Pe - Pb : 0) * ncb;
+    double ra[NL], rb[NL], rsg = 0.0;
+    __shared__ double Ss[2][TK];
+
+    // raw loads only (the k-block's signs go to LDS and are applied to the
+    // fragments), so the loads stay in flight across the current block's MFMAs
+    auto fetch = [&](int q) {
+        const int P = Pb + q / ncb, c = cb + (q % ncb) * TK;
+        const double* __restrict__ base = m + ((size_t)P * nk + c) * ldm;
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+            const int e = tid + r * kBlock, mm = e % T, k = e / T;
+            ra[r] = base[(size_t)k * ldm + mu0 + mm];
+            rb[r] = base[(size_t)k * ldm + nu0 + mm];
+        }
+        if (tid < TK) rsg = sign[c + tid];
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+            const int e = tid + r * kBlock;
+            As[buf][e / T][e % T] = ra[r];
+            Bs[buf][e / T][e % T] = rb[r];
+        }
+        if (tid < TK) Ss[buf][tid] = rsg;
+    };
 
     d4 acc[F][F];
 #pragma unroll
@@ -157,33 +218,34 @@ __global__ __launch_bounds__(kBlock) void k_ksyrk_f64(const double* __restrict__
 #pragma unroll
         for (int j = 0; j < F; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-    for (int P = Pb; P < Pe; ++P) {
-        for (int c = cb; c < ce; c += TK) {
-            const double* __restrict__ base = m + ((size_t)P * nk + c) * ldm;
+    if (nq > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    // branch-free body (one basic block keeps the accumulators in AGPRs):
+    // the last iteration re-fetches its own block and stashes it into the
+    // idle buffer, which nothing reads afterwards
+    for (int q = 0; q < nq; ++q) {
+        const int cur = q & 1;
+        fetch(q + 1 < nq ? q + 1 : q);
 #pragma unroll
-            for (int r = 0; r < TK * T / kBlock; ++r) {
-                int e = tid + r * kBlock;
-                int mm = e % T, k = e / T;
-                As[k][mm] = base[(size_t)k * ldm + mu0 + mm];
-                Bs[k][mm] = sign[c + k] * base[(size_t)k * ldm + nu0 + mm];
-            }
-            __syncthreads();
+        for (int kk = 0; kk < TK; kk += 4) {
+            double bmu[F], anu[F];
+            const double sg = Ss[cur][kk + (lane >> 4)];
 #pragma unroll
-            for (int kk = 0; kk < TK; kk += 4) {
-                double bmu[F], anu[F];
+            for (int i = 0; i < F; ++i) bmu[i] = As[cur][kk + (lane >> 4)][wm * F * 16 + i * 16 + (lane & 15)];
 #pragma unroll
-                for (int i = 0; i < F; ++i) bmu[i] = As[kk + (lane >> 4)][wm * F * 16 + i * 16 + (lane & 15)];
+            for (int j = 0; j < F; ++j) anu[j] = sg * Bs[cur][kk + (lane >> 4)][wn * F * 16 + j * 16 + (lane & 15)];
+            // D[ν][μ] = Σ_k sign_k M'[k][ν] M'[k][μ]
 #pragma unroll
-                for (int j = 0; j < F; ++j) anu[j] = Bs[kk + (lane >> 4)][wn * F * 16 + j * 16 + (lane & 15)];
-                // D[ν][μ] = Σ_k sign_k M'[k][ν] M'[k][μ]
+            for (int i = 0; i < F; ++i)
 #pragma unroll
-                for (int i = 0; i < F; ++i)
-#pragma unroll
-                    for (int j = 0; j < F; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(anu[j], bmu[i], acc[i][j], 0, 0, 0);
-            }
-            __syncthreads();
+                for (int j = 0; j < F; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(anu[j], bmu[i], acc[i][j], 0, 0, 0);
         }
+        stash(cur ^ 1);
+        __syncthreads();
     }
 
     double* __restrict__ kp = kpart + ((size_t)split * nspin + spin) * slot;

@@ -292,3 +292,47 @@ def test_random_orthonormal():
 
 def test_numpy_synth_equals_c_synth_other_shape():
     np.testing.assert_array_equal(orc.synth_rimatr(3, 7, 5), np_synth_rimatr(3, 7, 5))
+
+
+def _energy_functional(bfull, h, dms, uhf):
+    """RI-HF electronic energy from unpacked B, computed directly:
+    RHF  E = tr(DH) + ½Σ_P γ_P² − ¼Σ_P tr(D B_P D B_P)
+    UHF  E = Σ_s tr(D_s H) + ½Σ_P (γ_α+γ_β)² − ½Σ_s Σ_P tr(D_s B_P D_s B_P)"""
+    naux = bfull.shape[2]
+    e = sum(np.sum(d * h) for d in dms)
+    g = sum(np.einsum("ijP,ij->P", bfull, d) for d in dms)
+    e += 0.5 * np.dot(g, g)
+    kx = 0.0
+    for d in dms:
+        for P in range(naux):
+            bp = bfull[:, :, P]
+            kx += np.sum((bp @ d @ bp) * d)
+    return e - (0.5 if uhf else 0.25) * kx
+
+
+@pytest.mark.parametrize("uhf", [False, True])
+def test_fock_is_energy_gradient(uhf):
+    """F (H + J_0 + J_1 + s_K·K with s_K = −0.5 RHF / −1 UHF) is dE/dD of the RI
+    energy functional: checked by central finite differences."""
+    rng = np.random.default_rng(11 + uhf)
+    nbf, naux = 7, 9
+    npair = nbf * (nbf + 1) // 2
+    b = rng.standard_normal((npair, naux)) * 0.3
+    bfull = np.stack([unpack(b[:, P], nbf) for P in range(naux)], axis=2)
+    h = rng.standard_normal((nbf, nbf)); h = h + h.T
+    ns = 2 if uhf else 1
+    dms = []
+    for _ in range(ns):
+        d = rng.standard_normal((nbf, nbf)); dms.append(d + d.T)
+    sk = -1.0 if uhf else -0.5
+    vj = orc.vj_upper_rimatr(b, dms, ns, 1.0)
+    vk = orc.vk_upper_dm(b, True, dms, ns, sk)
+    f = orc.fock_upper(h[upper_idx(nbf)], vj, vk, ns)
+    for s in range(ns):
+        dd = rng.standard_normal((nbf, nbf)); dd = dd + dd.T
+        eps = 1e-5
+        dp = [x.copy() for x in dms]; dp[s] = dp[s] + eps * dd
+        dmn = [x.copy() for x in dms]; dmn[s] = dmn[s] - eps * dd
+        fd = (_energy_functional(bfull, h, dp, uhf) - _energy_functional(bfull, h, dmn, uhf)) / (2 * eps)
+        analytic = np.sum(unpack(f[s], nbf) * dd)
+        np.testing.assert_allclose(analytic, fd, rtol=1e-7, atol=1e-7)
