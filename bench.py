@@ -27,9 +27,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-FP64_MFMA_PEAK_TFS = 78.6      # MI355X dense FP64 matrix
+MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3}   # MI355X dense matrix peaks of the instruction used
 
-PHASE_KERNEL = {"j_gamma": "k_jgamma", "j_acc": "k_jacc", "k_half": "k_khalf_f64", "k_syrk": "k_ksyrk_f64"}
+PHASE_KERNEL = {"j_gamma": "k_jgamma", "j_acc": "k_jacc", "k_half": "k_khalf", "k_syrk": "k_ksyrk"}
 
 
 def parse():
@@ -38,8 +38,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="h2o64", choices=["h2o64", "h2o16"])
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
+                    help="fp64: plain FP64 path; fp32: FP32 copy of B, FP32 MFMA K, FP64 accumulation of J")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (counter calibration)")
     return ap.parse_args()
 
 
@@ -78,7 +82,7 @@ def cpu_baseline(shape, seed, c, d, threads):
 
 
 def load_traffic(kernel: str):
-    p = ROOT / "profiles" / f"traffic_{kernel}.json"
+    p = ROOT / "profiles" / f"traffic_{kernel}.json"  # written from rocprofv3 --pmc runs
     if p.exists():
         try:
             return json.loads(p.read_text()).get("traffic_bytes_per_launch")
@@ -106,8 +110,9 @@ def main():
 
     c = wl.random_orthonormal(seed, nbf, nocc)
     d = c @ c.T
+    prec = jk.PREC_FP64 if args.prec == "fp64" else jk.PREC_FP32
     ri = jk.DeviceRI(nbf, naux)
-    ri.fill_synthetic(seed)
+    ri.fill_synthetic(seed, jk.STORE_FP64 | (jk.STORE_FP32 if prec == jk.PREC_FP32 else 0))
     # a non-default torch stream (default priority) shared by the library and
     # the timing events: the null stream would hand the library back its own
     stream = torch.cuda.Stream(device=dev)
@@ -120,9 +125,19 @@ def main():
     vj_dev = torch.empty(shape.npair, dtype=torch.float64, device=dev)
     vk_dev = torch.empty(shape.npair, dtype=torch.float64, device=dev)
 
+    if args.calibrate_copy_mib:
+        nbytes = args.calibrate_copy_mib << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        ri.copy_dev(dst.data_ptr(), src.data_ptr(), nbytes)
+        ri.synchronize()
+        assert bool((dst == 1).all())
+        del src, dst
+
     def step():
-        ri.vj_dev(d_dev.data_ptr(), 1, 1.0, vj_dev.data_ptr())
-        ri.vk_occ_dev(c_dev.data_ptr(), nocc, [nocc - 1], occ_dev.data_ptr(), 1, 1.0, vk_dev.data_ptr())
+        ri.vj_dev(d_dev.data_ptr(), 1, 1.0, vj_dev.data_ptr(), prec=prec)
+        ri.vk_occ_dev(c_dev.data_ptr(), nocc, [nocc - 1], occ_dev.data_ptr(), 1, 1.0, vk_dev.data_ptr(), prec=prec)
 
     for _ in range(args.warmup):
         step()
@@ -146,15 +161,16 @@ def main():
 
     ms_step = ms_total / args.steps
     kf = wl.k_flops(shape)
-    jbytes = wl.jk_bytes(shape, 8)
+    s_b = 8 if args.prec == "fp64" else 4
+    jbytes = wl.jk_bytes(shape, s_b)
     per_kernel = {}
     for name, (ms, n) in phases.items():
         if n:
             per_kernel[name] = {"ms_per_step": ms / args.steps, "launches_per_step": n / args.steps,
                                 "avg_launch_ms": ms / n}
     # algorithmic work per step of each kernel
-    work = {"j_gamma": ("hbm", shape.npair * naux * 8 + 8 * shape.npair),
-            "j_acc": ("hbm", shape.npair * naux * 8 + 8 * shape.npair),
+    work = {"j_gamma": ("hbm", shape.npair * naux * s_b + 8 * shape.npair),
+            "j_acc": ("hbm", shape.npair * naux * s_b + 8 * shape.npair),
             "k_half": ("mfma", kf["half"]), "k_syrk": ("mfma", kf["syrk"])}
     dom = max((k for k in per_kernel if k in work), key=lambda k: per_kernel[k]["ms_per_step"])
     bound, amount = work[dom]
@@ -164,7 +180,7 @@ def main():
     if bound == "hbm":
         ach, peak, unit = per_launch / sec / 1e9, HBM_PEAK_GBS, "GB/s"
     else:
-        ach, peak, unit = per_launch / sec / 1e12, FP64_MFMA_PEAK_TFS, "TFLOP/s"
+        ach, peak, unit = per_launch / sec / 1e12, MFMA_PEAK_TFS[args.prec], "TFLOP/s"
     roofline = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
                 "traffic": load_traffic(PHASE_KERNEL[dom]), "kernel": PHASE_KERNEL[dom],
                 "algorithmic_per_launch": per_launch}
@@ -175,16 +191,17 @@ def main():
         "j_build_hbm_GBps": jbytes / (j_ms / 1e3) / 1e9 if j_ms else None,
         "j_build_hbm_frac": (jbytes / (j_ms / 1e3) / 1e9) / HBM_PEAK_GBS if j_ms else None,
         "k_build_TFLOPs": kf["total"] / (k_ms / 1e3) / 1e12 if k_ms else None,
-        "k_build_mfma_frac": (kf["total"] / (k_ms / 1e3) / 1e12) / FP64_MFMA_PEAK_TFS if k_ms else None,
+        "k_build_mfma_frac": (kf["total"] / (k_ms / 1e3) / 1e12) / MFMA_PEAK_TFS[args.prec] if k_ms else None,
         "per_kernel": per_kernel,
     }
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.prec == "fp64":
         cpu = cpu_baseline(shape, seed, c, d, args.cpu_threads)
 
     out = {
-        "metric": f"RI-V J/K builds/s (one J + one K), {shape.name} shape, plain FP64",
+        "metric": f"RI-V J/K builds/s (one J + one K), {shape.name} shape, "
+                  f"{'plain FP64' if args.prec == 'fp64' else 'FP32 B + FP32 MFMA'}",
         "value": args.steps / (ms_total / 1e3),
         "unit": "J/K builds/s",
         "n_gpus": 1,
@@ -195,7 +212,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.prec == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": f"{shape.name}: nbf={nbf} naux={naux} npair={shape.npair} nocc={nocc}, "
                                f"packed FP64 B resident in HBM (seed {seed}), D=C C^T, K occupied route",

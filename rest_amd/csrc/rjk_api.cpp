@@ -370,50 +370,46 @@ void k_plan_cols(KPlan& kp, size_t nbf, const size_t* ncol, int ns) {
     kp.nk = round_up(kp.col_cnt[0] + kp.col_cnt[1], kColPad);
 }
 
-// chunking and split-K; carves [cocc | sign | kpart | M'] out of the workspace
-int k_plan_ws(rjk_ctx* ctx, KPlan& kp, int ns, double** cocc, double** sign, double** kpart, double** m) {
+// chunking and split-K; carves [cocc | sign | kpart | M'] out of the workspace.
+// C', the signs and M' use the element type of the K precision; the split-K
+// partial tiles are FP64.
+int k_plan_ws(rjk_ctx* ctx, KPlan& kp, int ns, int prec, void** cocc, void** sign, double** kpart, void** m) {
+    const size_t es = k_elem_size(prec);
     const size_t nt = kp.nbf_pad / kTileMN;
     const size_t ntiles = nt * (nt + 1) / 2;
     const size_t target = 1024;  // SYRK work-groups
     const size_t slot = kp.nbf_pad * kp.nbf_pad;
     const size_t slot_bytes = (size_t)ns * slot * sizeof(double);
-    const size_t perP = kp.nk * kp.nbf_pad * sizeof(double);
-    const size_t small = round_up(kp.nbf_pad * kp.nk * sizeof(double), 256) + round_up(kp.nk * sizeof(double), 256);
-    // split-K slots: enough work-groups, at most a quarter of the workspace,
-    // and at least 8 auxiliary columns per slot in every chunk
-    // whole rounds of work-groups: floor, so that the last round is nearly full
-    size_t nsplit = std::max<size_t>(1, target / (ntiles * ns));
-    nsplit = std::min(nsplit, std::max<size_t>(1, (ctx->ws_limit / 4) / slot_bytes));
-    size_t chunk = 1;
-    for (int it = 0; it < 3; ++it) {
-        size_t fixed = small + nsplit * slot_bytes;
-        size_t avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
-        chunk = perP ? std::min(ctx->naux, std::max<size_t>(1, avail / perP)) : ctx->naux;
-        if (chunk > 65535) chunk = 65535;  // grid.z limit
-        size_t cap = std::max<size_t>(1, chunk / 8);
-        if (nsplit <= cap) break;
-        nsplit = cap;
-    }
+    const size_t perP = kp.nk * kp.nbf_pad * es;
+    const size_t small = round_up(kp.nbf_pad * kp.nk * es, 256) + round_up(kp.nk * es, 256);
+    // split-K slots: a multiple of 8 (one XCD per split), about `target`
+    // work-groups, at most a quarter of the workspace
+    size_t nsplit = round_up(std::max<size_t>(1, target / (ntiles * ns)), 8);
+    const size_t cap = std::max<size_t>(8, ((ctx->ws_limit / 4) / slot_bytes) / 8 * 8);
+    nsplit = std::min(nsplit, cap);
+    const size_t fixed = small + nsplit * slot_bytes;
+    const size_t avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
+    size_t chunk = perP ? std::min(ctx->naux, std::max<size_t>(1, avail / perP)) : ctx->naux;
     kp.nsplit = nsplit;
-    Carve cv;
-    size_t ic = cv.add(kp.nbf_pad * kp.nk * sizeof(double));
-    size_t is = cv.add(kp.nk * sizeof(double));
-    size_t ik = cv.add(kp.nsplit * ns * slot * sizeof(double));
     kp.chunk = chunk;
+    Carve cv;
+    size_t ic = cv.add(kp.nbf_pad * kp.nk * es);
+    size_t is = cv.add(kp.nk * es);
+    size_t ik = cv.add(kp.nsplit * ns * slot * sizeof(double));
     size_t im = cv.add(chunk * perP);
     void* ws = nullptr;
     int rc = ws_get(ctx, cv.total, &ws);
     if (rc) return rc;
     char* base = (char*)ws;
-    *cocc = (double*)(base + cv.off[ic]);
-    *sign = (double*)(base + cv.off[is]);
+    *cocc = base + cv.off[ic];
+    *sign = base + cv.off[is];
     *kpart = (double*)(base + cv.off[ik]);
-    *m = (double*)(base + cv.off[im]);
+    *m = base + cv.off[im];
     return RJK_OK;
 }
 
-int k_run(rjk_ctx* ctx, const KPlan& kp, int ns, int prec, const void* b, const double* cocc, const double* sign,
-          double* kpart, double* m, double scale, double* vk_dev) {
+int k_run(rjk_ctx* ctx, const KPlan& kp, int ns, int prec, const void* b, const void* cocc, const void* sign,
+          double* kpart, void* m, double scale, double* vk_dev) {
     hipStream_t st = ctx->stream;
     if (kp.nk == 0) {  // no occupied column (or D = 0): K = 0
         RJK_HIP(ctx, hipMemsetAsync(vk_dev, 0, (size_t)ns * ctx->npair * sizeof(double), st));
@@ -445,7 +441,6 @@ extern "C" int rjk_vk_upper_occ_dev(rjk_ctx* ctx, const double* eigv_dev, size_t
     const void* b = nullptr;
     int rc = check_common(ctx, spin_channel, prec, &b);
     if (rc) return rc;
-    if (prec != RJK_PREC_FP64) return fail(ctx, RJK_ERR_ARG, "vk: only RJK_PREC_FP64 is available in this build");
     const int ns = (int)spin_channel;
     size_t nw[2] = {0, 0};
     for (int s = 0; s < ns; ++s) {
@@ -456,12 +451,14 @@ extern "C" int rjk_vk_upper_occ_dev(rjk_ctx* ctx, const double* eigv_dev, size_t
     RJK_HIP(ctx, hipSetDevice(ctx->device));
     KPlan kp;
     k_plan_cols(kp, ctx->nbf, nw, ns);
-    double *cocc, *sign, *kpart, *m;
-    rc = k_plan_ws(ctx, kp, ns, &cocc, &sign, &kpart, &m);
+    void *cocc, *sign, *m;
+    double* kpart;
+    rc = k_plan_ws(ctx, kp, ns, prec, &cocc, &sign, &kpart, &m);
     if (rc) return rc;
     {
         PhaseScope ps(ctx, 4);
-        RJK_HIP(ctx, launch_prep_cocc(ctx->stream, eigv_dev, ctx->nbf, num_state, nw, occ_dev, ns, kp, cocc, sign));
+        RJK_HIP(ctx, launch_prep_cocc(ctx->stream, prec, eigv_dev, ctx->nbf, num_state, nw, occ_dev, ns, kp, cocc,
+                                      sign));
     }
     return k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, vk_dev);
 }
@@ -515,7 +512,6 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
     const void* b = nullptr;
     int rc = check_common(ctx, spin_channel, prec, &b);
     if (rc) return rc;
-    if (prec != RJK_PREC_FP64) return fail(ctx, RJK_ERR_ARG, "vk: only RJK_PREC_FP64 is available in this build");
     const size_t nbf = ctx->nbf, n2 = nbf * nbf;
     const int ns = (int)spin_channel;
     rc = check_host_finite(ctx, dm, spin_channel * n2, "dm");
@@ -577,18 +573,20 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
     }
     KPlan kp;
     k_plan_cols(kp, nbf, ncol, ns);
-    double *cocc, *sign, *kpart, *m;
-    rc = k_plan_ws(ctx, kp, ns, &cocc, &sign, &kpart, &m);
+    void *cocc, *sign, *m;
+    double* kpart;
+    rc = k_plan_ws(ctx, kp, ns, prec, &cocc, &sign, &kpart, &m);
     if (rc) return rc;
-    RJK_HIP(ctx, hipMemsetAsync(cocc, 0, kp.nbf_pad * kp.nk * sizeof(double), st));
-    RJK_HIP(ctx, hipMemsetAsync(sign, 0, kp.nk * sizeof(double), st));
+    const size_t es = k_elem_size(prec);
+    RJK_HIP(ctx, hipMemsetAsync(cocc, 0, kp.nbf_pad * kp.nk * es, st));
+    RJK_HIP(ctx, hipMemsetAsync(sign, 0, kp.nk * es, st));
     for (int s = 0; s < ns; ++s) {
         if (!ncol[s]) continue;
         RJK_HIP(ctx, hipMemcpyAsync(d_idx + s * nbf, sel_idx[s].data(), ncol[s] * sizeof(int), hipMemcpyHostToDevice,
                                     st));
         RJK_HIP(ctx, hipMemcpyAsync(d_sel + s * nbf, sel_lam[s].data(), ncol[s] * sizeof(double),
                                     hipMemcpyHostToDevice, st));
-        RJK_HIP(ctx, launch_prep_cocc_idx(st, d_vec + s * n2, nbf, d_idx + s * nbf, d_sel + s * nbf, ncol[s],
+        RJK_HIP(ctx, launch_prep_cocc_idx(st, prec, d_vec + s * n2, nbf, d_idx + s * nbf, d_sel + s * nbf, ncol[s],
                                           kp.col_off[s], kp, cocc, sign));
     }
     rc = k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, d_out);

@@ -95,16 +95,17 @@ struct KPlan {
     size_t chunk = 0;        // P per chunk
     size_t nsplit = 0;       // split-K slots of the SYRK
 };
-hipError_t launch_prep_cocc(hipStream_t st, const double* eigv, size_t nbf, size_t nstate,
-                            const size_t* nw, const double* occ, int nspin, const KPlan& kp,
-                            double* cocc, double* sign);
-hipError_t launch_prep_cocc_idx(hipStream_t st, const double* vecs, size_t nbf, const int* idx,
-                                const double* lam, size_t ncol, size_t col_off, const KPlan& kp,
-                                double* cocc, double* sign);
-hipError_t launch_khalf(hipStream_t st, int prec, const void* b, size_t ldb, size_t nbf, size_t P0,
-                        size_t Pc, const KPlan& kp, const void* cocc, void* m);
+size_t k_elem_size(int prec);   // element size of C', M' and signs for a K precision
+hipError_t launch_prep_cocc(hipStream_t st, int prec, const double* eigv, size_t nbf, size_t nstate,
+                            const size_t* nw, const double* occ, int nspin, const KPlan& kp, void* cocc,
+                            void* sign);
+hipError_t launch_prep_cocc_idx(hipStream_t st, int prec, const double* vecs, size_t nbf, const int* idx,
+                                const double* lam, size_t ncol, size_t col_off, const KPlan& kp, void* cocc,
+                                void* sign);
+hipError_t launch_khalf(hipStream_t st, int prec, const void* b, size_t ldb, size_t nbf, size_t P0, size_t Pc,
+                        const KPlan& kp, const void* cocc, void* m);
 hipError_t launch_ksyrk(hipStream_t st, int prec, const void* m, size_t Pc, const KPlan& kp, int nspin,
-                        const double* sign, int beta, double* kpart);
+                        const void* sign, int beta, double* kpart);
 hipError_t launch_kreduce(hipStream_t st, const double* kpart, const KPlan& kp, int nspin, size_t nbf,
                           double scale, double* vk);
 

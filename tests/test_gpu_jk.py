@@ -205,6 +205,28 @@ def test_fp32_copy_j_error_is_stated():
         assert rel < 5e-6
 
 
+def test_fp32_k_error_is_stated():
+    """FP32 B copy, FP32 C', FP32 MFMA with FP32 accumulation per split-K slot,
+    FP64 partial tiles: relative max error vs the FP64 oracle is stated."""
+    nbf, naux, nocc = 130, 333, 17
+    say(f"FP32 K nbf={nbf} naux={naux} nocc={nocc}")
+    b = orc.synth_rimatr(21, nbf, naux)
+    c = orc.random_orthonormal(5, nbf, nocc)
+    occ = np.full(nocc, 2.0)
+    with jk.DeviceRI(nbf, naux) as ri:
+        ri.fill_synthetic(21, jk.STORE_FP64 | jk.STORE_FP32)
+        k32 = ri.vk_occ([c], [nocc - 1], [occ], 1, -0.5, prec=jk.PREC_FP32)[0]
+        k64 = ri.vk_occ([c], [nocc - 1], [occ], 1, -0.5)[0]
+        ref = orc.vk_upper_occ(b, True, [c], [nocc - 1], [occ], 1, -0.5)[0]
+        assert maxabs(k64, ref) <= TOL
+        rel = maxabs(k32, ref) / np.abs(ref).max()
+        say(f"  FP32 K relative max error {rel:.2e}")
+        assert rel < 2e-5
+        d = (c * occ) @ c.T
+        kd32 = ri.vk_dm([d], 1, -0.5, prec=jk.PREC_FP32)[0]
+        assert maxabs(kd32, ref) / np.abs(ref).max() < 2e-5
+
+
 def test_error_paths():
     say("error paths")
     nbf, naux = 6, 4
