@@ -105,6 +105,9 @@ int rjk_set_ri3fn(rjk_ctx* ctx, const double* ri3fn, unsigned store_mask);
 int rjk_fill_rimatr_synthetic(rjk_ctx* ctx, uint64_t seed, unsigned store_mask);
 /* Device pointer of the stored FP64 B (for read-only inspection). */
 const double* rjk_rimatr_dev(const rjk_ctx* ctx);
+/* Copy the stored FP64 B back to the host in REST's rimatr layout
+ * [naux][npair] (the SCF.rimatr field, src/scf_io/mod.rs:50-51). */
+int rjk_get_rimatr(rjk_ctx* ctx, double* rimatr);
 
 /* ---- J (Coulomb) -------------------------------------------------------- */
 

@@ -1,0 +1,72 @@
+/*
+ * rest_scf.h — C ABI of the device-resident RI-HF SCF driver around the
+ * J/K build (closed shell).
+ *
+ * Follows REST's `scf()` (src/scf_io/mod.rs:3190-3268) for RHF with RI-V:
+ * guess density → Fock (generate_hf_hamiltonian_ri_v, :1339-1381; energy
+ * :1715-1802) → diagonalise F C = S C ε (diagonalize_hamiltonian, :1843) →
+ * density (generate_density_matrix, :441-478) → loop with linear mixing
+ * before start_diis_cycle and Pulay DIIS on e = F D S − S D F afterwards
+ * (prepare_next_input, :2947-3091; diis_solver, :3138-3188), until |ΔE|,
+ * ‖ΔD‖_F/nbf and ‖Δε‖₂ meet the thresholds (check_scf_convergence, :1875-1951).
+ *
+ * All matrices stay in HBM; J/K use the context's resident B.  With
+ * jk_prec = RJK_PREC_FP32 the J/K builds run on the FP32 copy of B until the
+ * DIIS error max|FDS − SDF| falls below switch_diis_err, then switch to FP64
+ * for the remaining iterations (the FP64 builds act as iterative
+ * refinement); convergence is only declared on two consecutive FP64 energies.
+ * Diagonalisation, DIIS, the density and the energy are FP64 throughout.
+ */
+#ifndef REST_SCF_H
+#define REST_SCF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rest_jk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rscf_ctrl {
+    int num_max_diis;        /* REST num_max_diis (8 in the examples)      */
+    int start_diis_cycle;    /* REST start_diis_cycle (1)                  */
+    double mix_param;        /* REST mix_param (0.6)                       */
+    int max_scf_cycle;       /* REST max_scf_cycle (100)                   */
+    double scf_acc_rho;      /* REST scf_acc_rho (1e-5)                    */
+    double scf_acc_eev;      /* REST scf_acc_eev (1e-5)                    */
+    double scf_acc_etot;     /* REST scf_acc_etot (1e-8)                   */
+    int jk_prec;             /* RJK_PREC_FP64 (plain) or RJK_PREC_FP32 (mixed) */
+    double switch_diis_err;  /* mixed mode: switch to FP64 below this max|e| (1e-5) */
+    int print_level;         /* 0 silent, 1 one line per iteration on stdout */
+} rscf_ctrl;
+
+/* Per-iteration record fields (iter_log rows of RSCF_LOG_FIELDS doubles). */
+enum { RSCF_LOG_E = 0, RSCF_LOG_DE, RSCF_LOG_DIIS_MAX, RSCF_LOG_DM, RSCF_LOG_EEV, RSCF_LOG_PREC,
+       RSCF_LOG_JK_MS, RSCF_LOG_DIAG_MS, RSCF_LOG_ITER_MS, RSCF_LOG_FIELDS };
+
+typedef struct rscf_result {
+    double energy;           /* final total energy (Eh)                    */
+    int converged;
+    int iterations;          /* SCF iterations after the guess step        */
+    int fp64_iterations;     /* of which with FP64 J/K                     */
+    double wall_seconds;     /* whole driver, guess Fock build included    */
+    double jk_seconds, diag_seconds, diis_seconds;
+} rscf_result;
+
+void rscf_default_ctrl(rscf_ctrl* ctrl);
+
+/* Closed-shell RI-HF.  S, H: nbf×nbf column-major host matrices (overlap and
+ * core Hamiltonian); enuc: nuclear repulsion; nelec: electron count (even);
+ * d_guess: host nbf×nbf guess density or NULL for the core-Hamiltonian guess.
+ * Outputs (each may be NULL): mo_coeff nbf×nbf, mo_energy nbf, dm nbf×nbf,
+ * iter_log max_scf_cycle × RSCF_LOG_FIELDS (row-major). */
+int rscf_rhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nelec, const double* d_guess,
+             const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff, double* mo_energy, double* dm,
+             double* iter_log);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REST_SCF_H */

@@ -23,12 +23,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-I", str(PKG.parent / "include")]
-SOURCES = ["rjk_api.cpp", "rjk_j.hip", "rjk_k.hip", "rjk_util.hip"]
+SOURCES = ["rjk_api.cpp", "rjk_j.hip", "rjk_k.hip", "rjk_util.hip", "rsi_int.cpp", "rsi_build.hip", "rscf.hip"]
 # MFMA accumulators in VGPRs: the K tiles carry 64-128 accumulator registers
 # across a software-pipelined loop; the default AGPR form makes the allocator
 # copy them VGPR<->AGPR every iteration.
-PER_FILE = {"rjk_k.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
-LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
+PER_FILE = {"rjk_k.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"], "rsi_int.cpp": ["-fopenmp"]}
+LIBS = ["-fopenmp", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 
 
 def _compile(src: Path) -> Path:
@@ -49,7 +49,7 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError("hipcc not found")
     OBJ.mkdir(parents=True, exist_ok=True)
     srcs = [CSRC / s for s in SOURCES]
-    with cf.ThreadPoolExecutor(max_workers=min(4, len(srcs))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(6, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs), *LIBS]

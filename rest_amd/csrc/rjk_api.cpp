@@ -150,7 +150,7 @@ extern "C" int rjk_set_workspace_limit(rjk_ctx* ctx, size_t bytes) {
 // ---------------------------------------------------------------------------
 // B storage
 // ---------------------------------------------------------------------------
-namespace {
+namespace rjk {
 
 int alloc_b(rjk_ctx* ctx, unsigned mask) {
     if (!(mask & RJK_STORE_FP64))
@@ -186,7 +186,7 @@ int finish_b(rjk_ctx* ctx) {
     return RJK_OK;
 }
 
-}  // namespace
+}  // namespace rjk
 
 extern "C" int rjk_set_rimatr(rjk_ctx* ctx, const double* rimatr, unsigned store_mask) {
     if (!ctx || !rimatr) return ctx ? fail(ctx, RJK_ERR_ARG, "rimatr is NULL") : RJK_ERR_ARG;
@@ -237,6 +237,17 @@ extern "C" int rjk_fill_rimatr_synthetic(rjk_ctx* ctx, uint64_t seed, unsigned s
 }
 
 extern "C" const double* rjk_rimatr_dev(const rjk_ctx* ctx) { return ctx ? ctx->b64 : nullptr; }
+
+extern "C" int rjk_get_rimatr(rjk_ctx* ctx, double* rimatr) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!rimatr) return fail(ctx, RJK_ERR_ARG, "rjk_get_rimatr: NULL output");
+    if (!ctx->b64) return fail(ctx, RJK_ERR_STATE, "B is not set");
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    RJK_HIP(ctx, hipMemcpy2DAsync(rimatr, ctx->npair * sizeof(double), ctx->b64, ctx->ldb * sizeof(double),
+                                  ctx->npair * sizeof(double), ctx->naux, hipMemcpyDeviceToHost, ctx->stream));
+    RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RJK_OK;
+}
 
 // ---------------------------------------------------------------------------
 // J

@@ -63,6 +63,9 @@ int fail(rjk_ctx* ctx, int code, const std::string& msg);
 
 // workspace: returns a device buffer of at least `bytes` (grow-only)
 int ws_get(rjk_ctx* ctx, size_t bytes, void** out);
+// device B storage: allocate the copies `mask` asks for; then refresh the FP32 copy and synchronise
+int alloc_b(rjk_ctx* ctx, unsigned mask);
+int finish_b(rjk_ctx* ctx);
 
 // phase timing around a launch
 struct PhaseScope {

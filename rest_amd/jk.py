@@ -41,10 +41,15 @@ ERRORS = {1: "ARG", 2: "HIP", 3: "NOMEM", 4: "STATE", 5: "NONFINITE", 6: "NOSOLV
 EXPORTS = [
     "rjk_create", "rjk_destroy", "rjk_last_error", "rjk_num_basis", "rjk_num_auxbas", "rjk_num_baspar",
     "rjk_ld_baspar", "rjk_set_stream", "rjk_synchronize", "rjk_set_workspace_limit", "rjk_set_rimatr",
-    "rjk_set_rimatr_dev", "rjk_set_ri3fn", "rjk_fill_rimatr_synthetic", "rjk_rimatr_dev",
+    "rjk_set_rimatr_dev", "rjk_set_ri3fn", "rjk_fill_rimatr_synthetic", "rjk_rimatr_dev", "rjk_get_rimatr",
     "rjk_vj_upper_with_rimatr_v_sync", "rjk_vj_upper_with_ri_v_sync", "rjk_vj_upper_dev",
     "rjk_vk_upper_with_ri_v_sync", "rjk_vk_upper_occ_dev", "rjk_vk_upper_with_rimatr_use_dm_only_sync",
     "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_profile_enable", "rjk_profile_read", "rjk_copy_dev",
+    # include/rest_ri.h
+    "rsi_basis_create", "rsi_basis_destroy", "rsi_basis_nbf", "rsi_last_error", "rsi_int1e", "rsi_int2c2e",
+    "rsi_int3c2e_packed", "rsi_build_rimatr",
+    # include/rest_scf.h
+    "rscf_default_ctrl", "rscf_rhf",
 ]
 
 
@@ -90,6 +95,7 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rjk_set_ri3fn": (i, [vp, vp, C.c_uint]),
         "rjk_fill_rimatr_synthetic": (i, [vp, C.c_uint64, C.c_uint]),
         "rjk_rimatr_dev": (vp, [vp]),
+        "rjk_get_rimatr": (i, [vp, vp]),
         "rjk_vj_upper_with_rimatr_v_sync": (i, [vp, vp, sz, d, i, vp]),
         "rjk_vj_upper_with_ri_v_sync": (i, [vp, vp, sz, d, i, vp]),
         "rjk_vj_upper_dev": (i, [vp, vp, sz, d, i, i, vp]),
@@ -100,6 +106,14 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rjk_profile_enable": (i, [vp, i]),
         "rjk_profile_read": (i, [vp, pd, C.POINTER(C.c_uint64)]),
         "rjk_copy_dev": (i, [vp, vp, vp, sz]),
+        "rsi_basis_create": (i, [sz, vp, vp, vp, vp, vp, sz, vp, C.POINTER(vp)]),
+        "rsi_basis_destroy": (i, [vp]),
+        "rsi_basis_nbf": (sz, [vp]),
+        "rsi_last_error": (C.c_char_p, []),
+        "rsi_int1e": (i, [vp, sz, vp, vp, vp, vp, vp]),
+        "rsi_int2c2e": (i, [vp, vp]),
+        "rsi_int3c2e_packed": (i, [vp, vp, sz, sz, vp]),
+        "rsi_build_rimatr": (i, [vp, vp, vp, d, C.c_uint, C.POINTER(sz), pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -186,6 +200,12 @@ class DeviceRI:
 
     def set_rimatr_dev(self, ptr: int, store: int = STORE_FP64):
         self._check(self.lib.rjk_set_rimatr_dev(self.h, C.c_void_p(ptr), store))
+
+    def get_rimatr(self) -> np.ndarray:
+        """Stored FP64 B as a host array indexed [p, P] (REST's rimatr)."""
+        buf = np.empty((self.naux, self.npair))
+        self._check(self.lib.rjk_get_rimatr(self.h, _ptr(buf)))
+        return buf.T
 
     def rimatr_dev_ptr(self) -> int:
         return int(self.lib.rjk_rimatr_dev(self.h) or 0)

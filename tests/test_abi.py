@@ -13,9 +13,11 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def header_symbols():
-    text = (ROOT / "include" / "rest_jk.h").read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(rjk_[a-z0-9_]+)\s*\(", text)))
+    syms = set()
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        syms |= set(re.findall(r"\b((?:rjk|rsi|rscf)_[a-z0-9_]+)\s*\(", text))
+    return sorted(syms)
 
 
 def test_library_exports_every_header_symbol():
