@@ -58,13 +58,21 @@ int rsi_int2c2e(const rsi_basis* aux, double* V);
  * orbital basis (j0, j1 on shell boundaries), all P: out[P][q] with
  * q = p - j0(j0+1)/2 the packed pair index relative to the slab. */
 int rsi_int3c2e_packed(const rsi_basis* orb, const rsi_basis* aux, size_t j0, size_t j1, double* out);
+/* The same slab computed by the device kernels on ctx's device and stream
+ * into the device array out_dev[naux][nq]; synchronises before returning.
+ * Orbital shells up to d and auxiliary shells up to g (RJK_ERR_ARG beyond).
+ * REST has no device counterpart; it replaces the libcint int3c2e calls of
+ * prepare_rimatr_for_ri_v_rayon_v02 (src/molecule_io/mod.rs:1647). */
+int rsi_int3c2e_packed_dev(rjk_ctx* ctx, const rsi_basis* orb, const rsi_basis* aux, size_t j0, size_t j1,
+                           double* out_dev);
 
 /* Build packed B = (μν|Q)·V^{-1/2} into the FP64 storage of ctx (and its FP32
  * copy when store_mask asks for it): V^{-1/2} = U diag(λ^{-1/2}) Uᵀ over the
  * eigenpairs with λ > lin_dep_threshold (1e-8 in REST); the three-centre
- * integrals are computed on the host in pair slabs and multiplied on the
- * device.  nkept (may be NULL) receives the number of kept eigenvalues;
- * seconds (may be NULL) receives {3c integrals, V^-1/2, GEMM} wall times. */
+ * integrals are computed in pair slabs by the device kernels (on the host
+ * when an orbital shell is beyond d) and multiplied on the device.  nkept (may be NULL) receives the number of kept eigenvalues;
+ * seconds (may be NULL) receives {3c integrals, V^-1/2, GEMM tail} times (device
+ * kernel time for device integrals). */
 int rsi_build_rimatr(rjk_ctx* ctx, const rsi_basis* orb, const rsi_basis* aux, double lin_dep_threshold,
                      unsigned store_mask, size_t* nkept, double* seconds);
 

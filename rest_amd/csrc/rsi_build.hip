@@ -20,6 +20,7 @@
 
 #include "../../include/rest_ri.h"
 #include "rjk_internal.hpp"
+#include "rsi_gpu3c.hpp"
 #include "rsi_internal.hpp"
 
 namespace {
@@ -35,6 +36,10 @@ struct Host {
     ~Host() {
         if (p) (void)hipHostFree(p);
     }
+};
+struct GpuInts {
+    rsi::Gpu3c g;
+    ~GpuInts() { rsi::gpu3c_free(&g); }
 };
 struct Blas {
     rocblas_handle h = nullptr;
@@ -129,10 +134,10 @@ extern "C" int rsi_build_rimatr(rjk_ctx* ctx, const rsi_basis* orb, const rsi_ba
     hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((naux * naux + 255) / 256)), dim3(256), 0, st, (double*)dU.p,
                        (int)naux, (const double*)dw.p);
     RSI_HIP(ctx, hipGetLastError());
-    const double one = 1.0, zero = 0.0;
+    const double one_ = 1.0, zero_ = 0.0;
     RSI_BLAS(ctx, rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)naux,
-                                (rocblas_int)naux, (rocblas_int)naux, &one, (const double*)dU.p, (rocblas_int)naux,
-                                (const double*)dU.p, (rocblas_int)naux, &zero, (double*)dVm.p, (rocblas_int)naux));
+                                (rocblas_int)naux, (rocblas_int)naux, &one_, (const double*)dU.p, (rocblas_int)naux,
+                                (const double*)dU.p, (rocblas_int)naux, &zero_, (double*)dVm.p, (rocblas_int)naux));
     RSI_HIP(ctx, hipStreamSynchronize(st));
     (void)hipFree(dU.p);
     dU.p = nullptr;
@@ -155,48 +160,88 @@ extern "C" int rsi_build_rimatr(rjk_ctx* ctx, const rsi_basis* orb, const rsi_ba
     size_t maxq = 0;
     for (size_t k = 0; k + 1 < cuts.size(); ++k)
         maxq = std::max(maxq, cuts[k + 1] * (cuts[k + 1] + 1) / 2 - cuts[k] * (cuts[k] + 1) / 2);
-    Host hX[2];
-    Dev dX[2];
-    hipEvent_t done[2];
-    for (int b = 0; b < 2; ++b) {
-        RSI_HIP(ctx, hipHostMalloc(&hX[b].p, maxq * naux * sizeof(double), hipHostMallocDefault));
-        RSI_HIP(ctx, hipMalloc(&dX[b].p, maxq * naux * sizeof(double)));
-        RSI_HIP(ctx, hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
-        RSI_HIP(ctx, hipEventRecord(done[b], st));
-    }
     RSI_HIP(ctx, hipMemsetAsync(ctx->b64, 0, naux * ldb * sizeof(double), st));
+    const double one = 1.0, zero = 0.0;
     int status = RJK_OK;
-    for (size_t k = 0; k + 1 < cuts.size() && status == RJK_OK; ++k) {
-        const int b = (int)(k & 1);
-        const size_t j0 = cuts[k], j1 = cuts[k + 1];
-        const size_t q0 = j0 * (j0 + 1) / 2, nq = j1 * (j1 + 1) / 2 - q0;
-        if (hipEventSynchronize(done[b]) != hipSuccess) {   // host buffer b free again
-            status = rjk::fail(ctx, RJK_ERR_HIP, "hipEventSynchronize failed");
-            break;
+    if (rsi::gpu3c_supported(*orb, *aux)) {
+        // device integrals: slab k's kernels and GEMM follow each other on the stream
+        GpuInts g;
+        Dev dX;
+        int r = rsi::gpu3c_init(&g.g, *orb, *aux, cuts, st);
+        if (r) return rjk::fail(ctx, r, "rsi_build_rimatr: device integral set-up failed");
+        RSI_HIP(ctx, hipMalloc(&dX.p, maxq * naux * sizeof(double)));
+        std::vector<hipEvent_t> ev(2 * (cuts.size() - 1), nullptr);
+        for (auto& e : ev) RSI_HIP(ctx, hipEventCreate(&e));
+        for (size_t k = 0; k + 1 < cuts.size() && status == RJK_OK; ++k) {
+            const size_t q0 = cuts[k] * (cuts[k] + 1) / 2, nq = cuts[k + 1] * (cuts[k + 1] + 1) / 2 - q0;
+            (void)hipEventRecord(ev[2 * k], st);
+            r = rsi::gpu3c_slab(g.g, k, st, (double*)dX.p, naux);
+            (void)hipEventRecord(ev[2 * k + 1], st);
+            if (r) {
+                status = rjk::fail(ctx, r, "rsi_build_rimatr: three-centre kernel launch failed");
+                break;
+            }
+            if (rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nq,
+                              (rocblas_int)naux, (rocblas_int)naux, &one, (const double*)dX.p, (rocblas_int)nq,
+                              (const double*)dVm.p, (rocblas_int)naux, &zero, ctx->b64 + q0,
+                              (rocblas_int)ldb) != rocblas_status_success)
+                status = rjk::fail(ctx, RJK_ERR_HIP, "rocblas_dgemm(B slab) failed");
         }
-        double t1 = now();
-        rsi::int3c_slab(*orb, *aux, j0, j1, (double*)hX[b].p);
-        t_3c += now() - t1;
-        if (hipMemcpyAsync(dX[b].p, hX[b].p, nq * naux * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) {
-            status = rjk::fail(ctx, RJK_ERR_HIP, "hipMemcpyAsync(X slab) failed");
-            break;
+        const double t1 = now();
+        if (hipStreamSynchronize(st) != hipSuccess && status == RJK_OK)
+            status = rjk::fail(ctx, RJK_ERR_HIP, "rsi_build_rimatr: device integrals / GEMM failed");
+        if (status == RJK_OK)
+            for (size_t k = 0; k + 1 < cuts.size(); ++k) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) == hipSuccess) t_3c += 1e-3 * ms;
+            }
+        t_gemm = now() - t1;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    } else {
+        // host integrals (orbital shells beyond d): slab k+1 on the host overlaps slab k's upload and GEMM
+        Host hX[2];
+        Dev dX[2];
+        hipEvent_t done[2];
+        for (int b = 0; b < 2; ++b) {
+            RSI_HIP(ctx, hipHostMalloc(&hX[b].p, maxq * naux * sizeof(double), hipHostMallocDefault));
+            RSI_HIP(ctx, hipMalloc(&dX[b].p, maxq * naux * sizeof(double)));
+            RSI_HIP(ctx, hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+            RSI_HIP(ctx, hipEventRecord(done[b], st));
         }
-        // B[P][q0+q] = Σ_Q X[Q][q] Vm[Q][P]   (column-major: Bm(nq × naux, ld ldb) = Xm(nq × naux) Vm)
-        if (rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nq, (rocblas_int)naux,
-                          (rocblas_int)naux, &one, (const double*)dX[b].p, (rocblas_int)nq, (const double*)dVm.p,
-                          (rocblas_int)naux, &zero, ctx->b64 + q0, (rocblas_int)ldb) != rocblas_status_success) {
-            status = rjk::fail(ctx, RJK_ERR_HIP, "rocblas_dgemm(B slab) failed");
-            break;
+        for (size_t k = 0; k + 1 < cuts.size() && status == RJK_OK; ++k) {
+            const int b = (int)(k & 1);
+            const size_t j0 = cuts[k], j1 = cuts[k + 1];
+            const size_t q0 = j0 * (j0 + 1) / 2, nq = j1 * (j1 + 1) / 2 - q0;
+            if (hipEventSynchronize(done[b]) != hipSuccess) {   // host buffer b free again
+                status = rjk::fail(ctx, RJK_ERR_HIP, "hipEventSynchronize failed");
+                break;
+            }
+            double t1 = now();
+            rsi::int3c_slab(*orb, *aux, j0, j1, (double*)hX[b].p);
+            t_3c += now() - t1;
+            if (hipMemcpyAsync(dX[b].p, hX[b].p, nq * naux * sizeof(double), hipMemcpyHostToDevice, st) !=
+                hipSuccess) {
+                status = rjk::fail(ctx, RJK_ERR_HIP, "hipMemcpyAsync(X slab) failed");
+                break;
+            }
+            // B[P][q0+q] = Σ_Q X[Q][q] Vm[Q][P]   (column-major: Bm(nq × naux, ld ldb) = Xm(nq × naux) Vm)
+            if (rocblas_dgemm(blas.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nq,
+                              (rocblas_int)naux, (rocblas_int)naux, &one, (const double*)dX[b].p, (rocblas_int)nq,
+                              (const double*)dVm.p, (rocblas_int)naux, &zero, ctx->b64 + q0,
+                              (rocblas_int)ldb) != rocblas_status_success) {
+                status = rjk::fail(ctx, RJK_ERR_HIP, "rocblas_dgemm(B slab) failed");
+                break;
+            }
+            if (hipEventRecord(done[b], st) != hipSuccess) {
+                status = rjk::fail(ctx, RJK_ERR_HIP, "hipEventRecord failed");
+                break;
+            }
         }
-        if (hipEventRecord(done[b], st) != hipSuccess) {
-            status = rjk::fail(ctx, RJK_ERR_HIP, "hipEventRecord failed");
-            break;
-        }
+        double t2 = now();
+        (void)hipStreamSynchronize(st);
+        t_gemm = now() - t2;   // tail of the device work after the last host slab
+        for (int b = 0; b < 2; ++b) (void)hipEventDestroy(done[b]);
     }
-    double t2 = now();
-    (void)hipStreamSynchronize(st);
-    t_gemm = now() - t2;   // tail of the device work after the last host slab
-    for (int b = 0; b < 2; ++b) (void)hipEventDestroy(done[b]);
     if (status != RJK_OK) {
         rsi::set_error(ctx->err);
         return status;
@@ -207,5 +252,31 @@ extern "C" int rsi_build_rimatr(rjk_ctx* ctx, const rsi_basis* orb, const rsi_ba
         seconds[1] = t_v;
         seconds[2] = t_gemm;
     }
+    return rc;
+}
+
+extern "C" int rsi_int3c2e_packed_dev(rjk_ctx* ctx, const rsi_basis* orb, const rsi_basis* aux, size_t j0, size_t j1,
+                                      double* out_dev) {
+    if (!ctx || !orb || !aux || !out_dev) {
+        rsi::set_error("rsi_int3c2e_packed_dev: NULL argument");
+        return RJK_ERR_ARG;
+    }
+    if (j0 >= j1 || j1 > orb->nbf || !rsi::on_shell_boundary(*orb, j0) || !rsi::on_shell_boundary(*orb, j1)) {
+        rsi::set_error("rsi_int3c2e_packed_dev: [j0, j1) must be a non-empty range on shell boundaries");
+        return RJK_ERR_ARG;
+    }
+    if (!rsi::gpu3c_supported(*orb, *aux)) {
+        rsi::set_error("rsi_int3c2e_packed_dev: device kernels cover orbital shells up to d and auxiliary up to g");
+        return RJK_ERR_ARG;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        rsi::set_error("rsi_int3c2e_packed_dev: hipSetDevice failed");
+        return RJK_ERR_HIP;
+    }
+    GpuInts g;
+    int rc = rsi::gpu3c_init(&g.g, *orb, *aux, std::vector<size_t>{j0, j1}, ctx->stream);
+    if (!rc) rc = rsi::gpu3c_slab(g.g, 0, ctx->stream, out_dev, aux->nbf);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = RJK_ERR_HIP;
+    if (rc) rsi::set_error("rsi_int3c2e_packed_dev: device integrals failed (code " + std::to_string(rc) + ")");
     return rc;
 }

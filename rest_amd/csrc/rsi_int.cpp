@@ -55,7 +55,6 @@ double gto_norm(int l, double a) {
     return 1.0 / std::sqrt(g);
 }
 
-constexpr double kPairCut = 1e-16;   // primitive pairs with |c_a c_b| exp(-μ R²) below this are skipped
 }  // namespace
 
 void set_error(const std::string& m) { g_err = m; }

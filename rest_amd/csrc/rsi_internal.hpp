@@ -6,6 +6,8 @@
 
 namespace rsi {
 
+constexpr double kPairCut = 1e-16;   // primitive pairs with |c_a c_b| exp(-μ R²) below this are skipped
+
 struct Shell {
     int l = 0, nprim = 0, atom = 0, nsph = 0, off = 0;
     double A[3] = {0, 0, 0};

@@ -47,7 +47,7 @@ EXPORTS = [
     "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_profile_enable", "rjk_profile_read", "rjk_copy_dev",
     # include/rest_ri.h
     "rsi_basis_create", "rsi_basis_destroy", "rsi_basis_nbf", "rsi_last_error", "rsi_int1e", "rsi_int2c2e",
-    "rsi_int3c2e_packed", "rsi_build_rimatr",
+    "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr",
     # include/rest_scf.h
     "rscf_default_ctrl", "rscf_rhf",
 ]
@@ -113,6 +113,7 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rsi_int1e": (i, [vp, sz, vp, vp, vp, vp, vp]),
         "rsi_int2c2e": (i, [vp, vp]),
         "rsi_int3c2e_packed": (i, [vp, vp, sz, sz, vp]),
+        "rsi_int3c2e_packed_dev": (i, [vp, vp, vp, sz, sz, vp]),
         "rsi_build_rimatr": (i, [vp, vp, vp, d, C.c_uint, C.POINTER(sz), pd]),
     }
     for name, (res, args) in sig.items():

@@ -115,6 +115,17 @@ def int3c2e_packed(orb: Basis, aux: Basis, j0: int = 0, j1: Optional[int] = None
     return out.T
 
 
+def int3c2e_packed_dev(ri: "jk.DeviceRI", orb: Basis, aux: Basis, j0: int = 0, j1: Optional[int] = None):
+    """Device kernels' (μν|P) for pair columns [j0, j1) as a torch tensor on ri's device, shape [P, q]."""
+    import torch
+    j1 = orb.nbf if j1 is None else j1
+    nq = j1 * (j1 + 1) // 2 - j0 * (j0 + 1) // 2
+    out = torch.empty((aux.nbf, nq), dtype=torch.float64, device="cuda")
+    rc = jk.load_library().rsi_int3c2e_packed_dev(ri.h, orb.h, aux.h, j0, j1, out.data_ptr())
+    _check(rc, "rsi_int3c2e_packed_dev")
+    return out
+
+
 @dataclass
 class Molecule:
     symbols: list
