@@ -45,6 +45,13 @@ int rsi_basis_destroy(rsi_basis* b);
 size_t rsi_basis_nbf(const rsi_basis* b);
 const char* rsi_last_error(void);   /* thread-local message of the last failure */
 
+/* The engine's Cartesian -> real solid harmonic rows for angular momentum l
+ * (0..4): (2l+1) × (l+1)(l+2)/2 row-major, m = -l..l (p: x, y, z), Cartesian
+ * components ordered lx = l..0, then ly = l-lx..0.  Rows are not normalised
+ * (each shell rescales them to unit-norm functions).  No REST counterpart;
+ * it lets callers rotate density blocks in the engine's own convention. */
+int rsi_cart2sph(int l, double* out);
+
 /* One-electron matrices, full nbf×nbf column-major (NULL outputs are skipped):
  * overlap S, kinetic T, nuclear attraction V = -Σ_C Z_C <μ|1/|r-C||ν> over the
  * point charges (charge[c], xyz[3c..]). */

@@ -1,6 +1,6 @@
 /*
  * rest_scf.h — C ABI of the device-resident RI-HF SCF driver around the
- * J/K build (closed shell).
+ * J/K build (closed shell RHF and open shell UHF).
  *
  * Follows REST's `scf()` (src/scf_io/mod.rs:3190-3268) for RHF with RI-V:
  * guess density → Fock (generate_hf_hamiltonian_ri_v, :1339-1381; energy
@@ -65,6 +65,36 @@ void rscf_default_ctrl(rscf_ctrl* ctrl);
 int rscf_rhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nelec, const double* d_guess,
              const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff, double* mo_energy, double* dm,
              double* iter_log);
+
+/* Unrestricted RI-HF (REST's UHF branch of the same functions): nalpha and
+ * nbeta electrons with occupation 1, F_s = H + J[D_α + D_β] − K[D_s]
+ * (src/scf_io/mod.rs:1347-1381), E = E_nuc + ½ Σ_s Σ D_s (H + F_s)
+ * (:1673-1713), one DIIS over the concatenated α and β error vectors
+ * (:3094-3136) and convergence on both spin densities (:1875-1951).
+ * d_guess: 2 × nbf² (α then β) or NULL for the core-Hamiltonian guess.
+ * Outputs hold α then β: mo_coeff 2×nbf², mo_energy 2×nbf, dm 2×nbf². */
+int rscf_uhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nalpha, size_t nbeta,
+             const double* d_guess, const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff,
+             double* mo_energy, double* dm, double* iter_log);
+
+/* REST's density-guess step (initial_guess for sad / hdf5 densities,
+ * src/initial_guess/mod.rs:18-26, :54-64): F(D0) with the density-only J/K
+ * (generate_hf_hamiltonian_for_guess, src/scf_io/mod.rs:1657-1671), the
+ * guess energy E(D0) (evaluate_hf_total_energy, :1673-1713, printed by REST
+ * as "Initial guess HF energy"), F C = S C ε and the density D1 of those
+ * orbitals, which is where REST's scf() starts.  nspin 1: nelec[0] is the
+ * electron count, d0 and d1 are nbf²; nspin 2: nelec = {nα, nβ}, d0 and d1
+ * are 2×nbf².  prec: J/K arithmetic (RJK_PREC_FP64 as in REST, or
+ * RJK_PREC_FP32 to rank guesses cheaply).  mo_coeff / mo_energy may be NULL. */
+int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin, const size_t* nelec,
+                    const double* d0, int prec, double* energy, double* d1, double* mo_coeff, double* mo_energy);
+
+/* McWeeny purification of a closed-shell density in the S metric (for a
+ * predicted guess density; no REST counterpart): P = D/2,
+ * P <- 3 P S P − 2 P S P S P `iterations` times, D_out = 2P; electrons
+ * (may be NULL) receives 2 tr(P S).  nbf×nbf column-major host matrices;
+ * GEMMs in FP64 on the device. */
+int rscf_mcweeny(rjk_ctx* ctx, const double* S, const double* D, int iterations, double* D_out, double* electrons);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,7 @@
 """CPU restatement of REST's RI-HF SCF loop (test infrastructure only).
 
 Follows src/scf_io/mod.rs:
-  - scf()                      :3190-3268  (guess → F, E → diagonalise → loop)
+  - scf()                      :3190-3268  (guess → F, E → diagonalise → loop), RHF and UHF
   - prepare_next_input         :2947-3091  (linear mixing before start_diis_cycle, Pulay DIIS after)
   - generate_diis_error_vector :3094-3136  (e = F D S − S D F, spins concatenated)
   - diis_solver                :3138-3188  (c = B⁻¹1 / 1ᵀB⁻¹1 over at most num_max_diis vectors)
@@ -142,6 +142,87 @@ def rhf(S, H, rimatr, nelec: int, enuc: float, ctrl: ScfCtrl = ScfCtrl(), d_gues
             break
     return ScfResult(energy=E, converged=converged, iterations=num_iter - 1, energies=energies, eigenvalues=eps,
                      eigenvectors=C, density=D, fock=F)
+
+
+def uhf(S, H, rimatr, nalpha: int, nbeta: int, enuc: float, ctrl: ScfCtrl = ScfCtrl(), d_guess=None,
+        jk=None) -> ScfResult:
+    """Unrestricted RI-HF (REST's UHF branch): occupation 1 per spin,
+    F_s = H + J[D_α + D_β] − K[D_s] (s_K = −1, src/scf_io/mod.rs:1347-1381),
+    E = E_nuc + ½ Σ_s Σ D_s (H + F_s) (:1673-1713), DIIS error vectors of the two
+    spins concatenated (:3094-3136), convergence on both densities and on the
+    joint eigenvalue change (:1875-1951).  Densities and Fock matrices are
+    returned as [2, n, n]."""
+    n = S.shape[0]
+    nocc = (nalpha, nbeta)
+    jk = jk or oracle_jk(rimatr)
+
+    def fock_energy(Ds):
+        J, _ = jk(Ds[0] + Ds[1])
+        Fs, E = [], enuc
+        for s in range(2):
+            K = jk(Ds[s])[1] if nocc[s] else np.zeros((n, n))
+            Fs.append(H + J - K)
+            E += 0.5 * np.sum(Ds[s] * (H + Fs[s]))
+        return Fs, E
+
+    def diag(F):
+        return sla.eigh(F, S)
+
+    def density(C, s):
+        return C[:, :nocc[s]] @ C[:, :nocc[s]].T
+
+    if d_guess is None:   # core-Hamiltonian guess
+        _, C0 = diag(H)
+        d_guess = [density(C0, 0), density(C0, 1)]
+    D_in = [np.array(d) for d in d_guess]
+    Fs, E = fock_energy(D_in)
+    rec_E = E
+    eig = [diag(F) for F in Fs]
+    rec_eps = [e for e, _ in eig]
+    Ds = [density(eig[s][1], s) for s in range(2)]
+    rec_D = [D_in, Ds]
+    num_iter, energies = 1, [E]
+    errs, targets = [], []
+    start_diis = ctrl.start_diis_cycle
+    converged = False
+    mix = lambda: [rec_D[0][s] + ctrl.mix_param * (rec_D[1][s] - rec_D[0][s]) for s in range(2)]
+    while True:
+        if num_iter < start_diis:
+            Fs, E = fock_energy(mix())
+        else:
+            Fs, E = fock_energy(Ds)
+            if len(targets) == ctrl.num_max_diis:
+                targets.pop(0)
+                errs.pop(0)
+            e = np.concatenate([(Fs[s] @ rec_D[1][s] @ S - S @ rec_D[1][s] @ Fs[s]).ravel() for s in range(2)])
+            errs.append(e)
+            targets.append([F.copy() for F in Fs])
+            Bm = np.array([[np.dot(a, b) for b in errs] for a in errs])
+            try:
+                inv = np.linalg.inv(Bm)
+                if not np.all(np.isfinite(inv)):
+                    raise np.linalg.LinAlgError
+                c = inv.sum(axis=1) / inv.sum()
+                Fs = [sum(ci * t[s] for ci, t in zip(c, targets)) for s in range(2)]
+            except np.linalg.LinAlgError:
+                Fs, E = fock_energy(mix())
+                start_diis = num_iter + 8
+                errs, targets = [], []
+        energies.append(E)
+        eig = [diag(F) for F in Fs]
+        eps = [e for e, _ in eig]
+        Ds = [density(eig[s][1], s) for s in range(2)]
+        dE = E - rec_E
+        dm_err = max(np.sqrt(np.sum((Ds[s] - rec_D[1][s]) ** 2)) / n for s in range(2))
+        eev_err = np.sqrt(sum(np.sum((eps[s] - rec_eps[s]) ** 2) for s in range(2)))
+        converged = abs(dE) <= ctrl.scf_acc_etot and dm_err <= ctrl.scf_acc_rho and eev_err <= ctrl.scf_acc_eev
+        rec_E, rec_eps, rec_D = E, eps, [rec_D[1], Ds]
+        num_iter += 1
+        if converged or num_iter >= ctrl.max_scf_cycle:
+            break
+    return ScfResult(energy=E, converged=converged, iterations=num_iter - 1, energies=energies,
+                     eigenvalues=np.array(eps), eigenvectors=np.array([c for _, c in eig]), density=np.array(Ds),
+                     fock=np.array(Fs))
 
 
 def rimatr_from_integrals(X, Vmetric, threshold: float = 1e-8):

@@ -117,6 +117,11 @@ __global__ void k_mix(const double* __restrict__ A, const double* __restrict__ B
     if (t < n2) out[t] = A[t] + alpha * (B[t] - A[t]);
 }
 
+__global__ void k_add(const double* __restrict__ A, const double* __restrict__ B, size_t n2, double* __restrict__ out) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n2) out[t] = A[t] + B[t];
+}
+
 __global__ void k_fill(double* p, size_t n, double v) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -217,23 +222,45 @@ extern "C" void rscf_default_ctrl(rscf_ctrl* c) {
     c->print_level = 0;
 }
 
-extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, double enuc, size_t nelec,
-                        const double* d_guess, const rscf_ctrl* ctrl_in, rscf_result* result, double* mo_coeff,
-                        double* mo_energy, double* dm_out, double* iter_log) {
-    if (!ctx) return RJK_ERR_ARG;
-    if (!S_h || !H_h) return rjk::fail(ctx, RJK_ERR_ARG, "rscf_rhf: S and H are required");
+namespace {
+
+// One SCF driver for RHF (nspin 1: nel[0] electrons, occupation 2) and UHF
+// (nspin 2: nel[0] α and nel[1] β electrons, occupation 1).  With guess_only
+// it performs REST's density-guess step only: F(D0) by the density-only J/K,
+// E(D0), F C = S C ε and the new density, written to dm_out.
+int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h, double enuc, int nspin,
+             const size_t nel[2], const double* d_guess, const rscf_ctrl* ctrl_in, bool guess_only,
+             rscf_result* result, double* mo_coeff, double* mo_energy, double* dm_out, double* iter_log) {
+    if (!S_h || !H_h) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": S and H are required");
     rscf_ctrl ctrl;
     rscf_default_ctrl(&ctrl);
     if (ctrl_in) ctrl = *ctrl_in;
     const size_t n = ctx->nbf, n2 = n * n, npair = ctx->npair;
-    if (nelec == 0 || (nelec & 1) || nelec / 2 > n)
-        return rjk::fail(ctx, RJK_ERR_ARG, "rscf_rhf: nelec must be even, positive and <= 2 nbf");
+    size_t nocc[2] = {0, 0};
+    double occ_val, sk;
+    if (nspin == 1) {
+        if (nel[0] == 0 || (nel[0] & 1) || nel[0] / 2 > n)
+            return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": nelec must be even, positive and <= 2 nbf");
+        nocc[0] = nel[0] / 2;
+        occ_val = 2.0;
+        sk = -0.5;   // REST scaling for RHF exchange (src/scf_io/mod.rs:1347-1350)
+    } else {
+        if (nel[0] + nel[1] == 0 || nel[0] > n || nel[1] > n)
+            return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": need 0 < nalpha + nbeta and each <= nbf");
+        nocc[0] = nel[0];
+        nocc[1] = nel[1];
+        occ_val = 1.0;
+        sk = -1.0;
+    }
     if (ctrl.num_max_diis < 1 || ctrl.num_max_diis > 16 || ctrl.max_scf_cycle < 1)
-        return rjk::fail(ctx, RJK_ERR_ARG, "rscf_rhf: num_max_diis must be 1..16 and max_scf_cycle >= 1");
-    if (!ctx->b64) return rjk::fail(ctx, RJK_ERR_STATE, "rscf_rhf: B is not set");
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": num_max_diis must be 1..16 and max_scf_cycle >= 1");
+    if (!ctx->b64) return rjk::fail(ctx, RJK_ERR_STATE, std::string(who) + ": B is not set");
     if (ctrl.jk_prec == RJK_PREC_FP32 && !ctx->b32)
-        return rjk::fail(ctx, RJK_ERR_STATE, "rscf_rhf: mixed precision needs the FP32 copy of B (RJK_STORE_FP32)");
-    const size_t nocc = nelec / 2;
+        return rjk::fail(ctx, RJK_ERR_STATE,
+                         std::string(who) + ": mixed precision needs the FP32 copy of B (RJK_STORE_FP32)");
+    if (ctrl.jk_prec != RJK_PREC_FP64 && ctrl.jk_prec != RJK_PREC_FP32)
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64 or FP32");
+    if (guess_only && !d_guess) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": a guess density is required");
     const double t_start = now();
     SCF_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;
@@ -246,99 +273,133 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
     DevMem mem;
     double* S = mem.get(ctx, n2, &rc);
     double* H = mem.get(ctx, n2, &rc);
-    double* F = mem.get(ctx, n2, &rc);
-    double* Fx = mem.get(ctx, n2, &rc);
-    double* A = mem.get(ctx, n2, &rc);      // eigenvectors (C) after dsygvd
     double* Bw = mem.get(ctx, n2, &rc);     // S copy consumed by dsygvd
-    double* D = mem.get(ctx, n2, &rc);      // current density
-    double* Dp = mem.get(ctx, n2, &rc);     // previous density (REST records.density_matrix[0])
-    double* Dm = mem.get(ctx, n2, &rc);     // mixed density
     double* T1 = mem.get(ctx, n2, &rc);
     double* T2 = mem.get(ctx, n2, &rc);
-    double* W = mem.get(ctx, n, &rc);
+    double* Dt = mem.get(ctx, n2, &rc);     // total density for J (UHF)
     double* Ework = mem.get(ctx, n, &rc);
     double* vj = mem.get(ctx, npair, &rc);
-    double* vk = mem.get(ctx, npair, &rc);
-    double* occd = mem.get(ctx, n, &rc);
     double* partial = mem.get(ctx, kRedBlocks, &rc);
     double* info_d = mem.get(ctx, 1, &rc);
+    // per spin: Fock, extrapolated Fock, eigenvectors, densities (current, previous,
+    // mixed), eigenvalues, occupations, exchange
+    double *F[2], *Fx[2], *A[2], *D[2], *Dp[2], *Dm[2], *W[2], *occd[2], *vk[2];
+    for (int s = 0; s < nspin; ++s) {
+        F[s] = mem.get(ctx, n2, &rc);
+        Fx[s] = mem.get(ctx, n2, &rc);
+        A[s] = mem.get(ctx, n2, &rc);
+        D[s] = mem.get(ctx, n2, &rc);
+        Dp[s] = mem.get(ctx, n2, &rc);
+        Dm[s] = mem.get(ctx, n2, &rc);
+        W[s] = mem.get(ctx, n, &rc);
+        occd[s] = mem.get(ctx, n, &rc);
+        vk[s] = mem.get(ctx, npair, &rc);
+    }
     const int nd = ctrl.num_max_diis;
-    double* ering = mem.get(ctx, (size_t)nd * n2, &rc);
-    double* fring = mem.get(ctx, (size_t)nd * n2, &rc);
+    const size_t ev_len = (size_t)nspin * n2;   // error vector: spins back to back (REST :3133)
+    double* ering = mem.get(ctx, (size_t)nd * ev_len, &rc);
+    double* fring = mem.get(ctx, (size_t)nd * ev_len, &rc);
     if (rc) return rc;
     SCF_HIP(hipMemcpyAsync(S, S_h, n2 * sizeof(double), hipMemcpyHostToDevice, st));
     SCF_HIP(hipMemcpyAsync(H, H_h, n2 * sizeof(double), hipMemcpyHostToDevice, st));
-    {
+    for (int s = 0; s < nspin; ++s) {
         std::vector<double> occ(n, 0.0);
-        for (size_t i = 0; i < nocc; ++i) occ[i] = 2.0;
-        SCF_HIP(hipMemcpyAsync(occd, occ.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        for (size_t i = 0; i < nocc[s]; ++i) occ[i] = occ_val;
+        SCF_HIP(hipMemcpyAsync(occd[s], occ.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        SCF_HIP(hipStreamSynchronize(st));   // occ is a host temporary
     }
     const dim3 g2((unsigned)((n2 + 255) / 256)), b256(256);
-    const double one = 1.0, zero = 0.0, two = 2.0;
+    const double one = 1.0, zero = 0.0;
     std::vector<double> part(kRedBlocks);
     double t_jk = 0, t_diag = 0, t_diis = 0;
 
     auto reduce_sum = [&](double* out) -> int {
         SCF_HIP(hipMemcpyAsync(part.data(), partial, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
         SCF_HIP(hipStreamSynchronize(st));
-        double s = 0.0;
-        for (double v : part) s += v;
-        *out = s;
+        double v = 0.0;
+        for (double x : part) v += x;
+        *out = v;
         return RJK_OK;
     };
     auto reduce_max = [&](double* out) -> int {
         SCF_HIP(hipMemcpyAsync(part.data(), partial, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
         SCF_HIP(hipStreamSynchronize(st));
-        double s = 0.0;
-        for (double v : part) s = std::max(s, v);
-        *out = s;
+        double v = 0.0;
+        for (double x : part) v = std::max(v, x);
+        *out = v;
         return RJK_OK;
     };
-    // diagonalise Fm: A <- C, W <- ε (host copy in eps)
-    std::vector<double> eps(n), eps_prev(n);
-    auto diagonalise = [&](const double* Fm) -> int {
+    // diagonalise Fm for spin s: A[s] <- C, W[s] <- ε (host copy in eps[s])
+    std::vector<double> eps[2], eps_prev[2];
+    for (int s = 0; s < nspin; ++s) {
+        eps[s].assign(n, 0.0);
+        eps_prev[s].assign(n, 0.0);
+    }
+    auto diagonalise = [&](const double* Fm, int s) -> int {
         const double t0 = now();
-        SCF_HIP(hipMemcpyAsync(A, Fm, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        SCF_HIP(hipMemcpyAsync(A[s], Fm, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
         SCF_HIP(hipMemcpyAsync(Bw, S, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        SCF_BLAS(rocsolver_dsygvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, A,
-                                  (rocblas_int)n, Bw, (rocblas_int)n, W, Ework, (rocblas_int*)info_d));
+        SCF_BLAS(rocsolver_dsygvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n,
+                                  A[s], (rocblas_int)n, Bw, (rocblas_int)n, W[s], Ework, (rocblas_int*)info_d));
         int info = 0;
         SCF_HIP(hipMemcpyAsync(&info, info_d, sizeof(int), hipMemcpyDeviceToHost, st));
-        SCF_HIP(hipMemcpyAsync(eps.data(), W, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        SCF_HIP(hipMemcpyAsync(eps[s].data(), W[s], n * sizeof(double), hipMemcpyDeviceToHost, st));
         SCF_HIP(hipStreamSynchronize(st));
         if (info != 0) return rjk::fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsygvd info = " + std::to_string(info));
         t_diag += now() - t0;
         return RJK_OK;
     };
-    // D = 2 C_occ C_occᵀ from A
-    auto density = [&](double* Dout) -> int {
+    // D_s = occ · C_occ C_occᵀ from A[s] (generate_density_matrix, REST :441-478)
+    auto density = [&](int s, double* Dout) -> int {
+        if (nocc[s] == 0) {
+            hipLaunchKernelGGL(k_fill, g2, b256, 0, st, Dout, n2, 0.0);
+            SCF_HIP(hipGetLastError());
+            return RJK_OK;
+        }
         SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)n, (rocblas_int)n,
-                               (rocblas_int)nocc, &two, A, (rocblas_int)n, A, (rocblas_int)n, &zero, Dout,
-                               (rocblas_int)n));
+                               (rocblas_int)nocc[s], &occ_val, A[s], (rocblas_int)n, A[s], (rocblas_int)n, &zero,
+                               Dout, (rocblas_int)n));
         return RJK_OK;
     };
-    // F = H + J[Dsrc] − ½ K[Dsrc]; K by the occupied route when the density
-    // is the one of the current orbitals A, else by the density-only route.
-    auto fock = [&](const double* Dsrc, bool occ_route, int prec, double* E) -> int {
+    // F_s = H + J[Σ_s D_s] + sK·K[D_s]; K by the occupied route when the densities
+    // are those of the current orbitals A, else by the density-only route.
+    // E = E_nuc + ½ Σ_s Σ D_s (H + F_s)   (REST :1715-1782, :1673-1713)
+    auto fock = [&](double* const* Dsrc, bool occ_route, int prec, double* E) -> int {
         const double t0 = now();
-        SCF_RC(rjk_vj_upper_dev(ctx, Dsrc, 1, 1.0, prec, 0, vj));
-        if (occ_route) {
-            const size_t homo[2] = {nocc - 1, 0};
-            SCF_RC(rjk_vk_upper_occ_dev(ctx, A, n, homo, occd, 1, -0.5, prec, vk));
-        } else {
-            std::vector<double> dh(n2), vkh(npair);
-            SCF_HIP(hipMemcpyAsync(dh.data(), Dsrc, n2 * sizeof(double), hipMemcpyDeviceToHost, st));
-            SCF_HIP(hipStreamSynchronize(st));
-            SCF_RC(rjk_vk_upper_with_rimatr_use_dm_only_sync(ctx, dh.data(), 1, -0.5, prec, vkh.data()));
-            SCF_HIP(hipMemcpyAsync(vk, vkh.data(), npair * sizeof(double), hipMemcpyHostToDevice, st));
+        const double* Djk = Dsrc[0];
+        if (nspin == 2) {
+            hipLaunchKernelGGL(k_add, g2, b256, 0, st, Dsrc[0], Dsrc[1], n2, Dt);   // D_α + D_β
+            SCF_HIP(hipGetLastError());
+            Djk = Dt;
         }
-        hipLaunchKernelGGL(k_fock, g2, b256, 0, st, H, vj, vk, n, 1.0, F);
-        SCF_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_energy, dim3(kRedBlocks), b256, 0, st, Dsrc, H, F, n2, partial);
-        SCF_HIP(hipGetLastError());
-        double s = 0.0;
-        SCF_RC(reduce_sum(&s));
-        *E = enuc + 0.5 * s;
+        SCF_RC(rjk_vj_upper_dev(ctx, Djk, 1, 1.0, prec, 0, vj));
+        for (int s = 0; s < nspin; ++s) {
+            if (nocc[s] == 0) {
+                hipLaunchKernelGGL(k_fill, dim3((unsigned)((npair + 255) / 256)), b256, 0, st, vk[s], npair, 0.0);
+                SCF_HIP(hipGetLastError());
+            } else if (occ_route) {
+                const size_t homo[2] = {nocc[s] - 1, 0};
+                SCF_RC(rjk_vk_upper_occ_dev(ctx, A[s], n, homo, occd[s], 1, sk, prec, vk[s]));
+            } else {
+                std::vector<double> dh(n2), vkh(npair);
+                SCF_HIP(hipMemcpyAsync(dh.data(), Dsrc[s], n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+                SCF_HIP(hipStreamSynchronize(st));
+                SCF_RC(rjk_vk_upper_with_rimatr_use_dm_only_sync(ctx, dh.data(), 1, sk, prec, vkh.data()));
+                SCF_HIP(hipMemcpyAsync(vk[s], vkh.data(), npair * sizeof(double), hipMemcpyHostToDevice, st));
+                SCF_HIP(hipStreamSynchronize(st));   // vkh is a host temporary
+            }
+        }
+        double e = 0.0;
+        for (int s = 0; s < nspin; ++s) {
+            hipLaunchKernelGGL(k_fock, g2, b256, 0, st, H, vj, vk[s], n, 1.0, F[s]);
+            SCF_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_energy, dim3(kRedBlocks), b256, 0, st, Dsrc[s], H, F[s], n2, partial);
+            SCF_HIP(hipGetLastError());
+            double v = 0.0;
+            SCF_RC(reduce_sum(&v));
+            e += v;
+        }
+        *E = enuc + 0.5 * e;
         t_jk += now() - t0;
         return RJK_OK;
     };
@@ -346,21 +407,42 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
     // ---- guess --------------------------------------------------------------
     bool guess_from_orbitals = false;
     if (d_guess) {
-        SCF_HIP(hipMemcpyAsync(Dp, d_guess, n2 * sizeof(double), hipMemcpyHostToDevice, st));
+        for (int s = 0; s < nspin; ++s)
+            SCF_HIP(hipMemcpyAsync(Dp[s], d_guess + (size_t)s * n2, n2 * sizeof(double), hipMemcpyHostToDevice, st));
     } else {   // core-Hamiltonian guess
-        SCF_RC(diagonalise(H));
-        SCF_RC(density(Dp));
+        for (int s = 0; s < nspin; ++s) {
+            SCF_RC(diagonalise(H, s));
+            SCF_RC(density(s, Dp[s]));
+        }
         guess_from_orbitals = true;
     }
     int prec = ctrl.jk_prec;
     double E = 0.0;
     SCF_RC(fock(Dp, guess_from_orbitals, prec, &E));
     int prec_E = prec;
-    SCF_RC(diagonalise(F));
-    SCF_RC(density(D));
+    for (int s = 0; s < nspin; ++s) {
+        SCF_RC(diagonalise(F[s], s));
+        SCF_RC(density(s, D[s]));
+    }
+    if (guess_only) {
+        SCF_HIP(hipStreamSynchronize(st));
+        if (result) {
+            *result = rscf_result{};
+            result->energy = E;
+            result->wall_seconds = now() - t_start;
+            result->jk_seconds = t_jk;
+            result->diag_seconds = t_diag;
+        }
+        for (int s = 0; s < nspin; ++s) {
+            if (dm_out) SCF_HIP(hipMemcpy(dm_out + (size_t)s * n2, D[s], n2 * sizeof(double), hipMemcpyDeviceToHost));
+            if (mo_coeff) SCF_HIP(hipMemcpy(mo_coeff + (size_t)s * n2, A[s], n2 * sizeof(double), hipMemcpyDeviceToHost));
+            if (mo_energy) std::memcpy(mo_energy + (size_t)s * n, eps[s].data(), n * sizeof(double));
+        }
+        return RJK_OK;
+    }
     double E_prev = E;
     int prec_prev = prec_E;
-    eps_prev = eps;
+    for (int s = 0; s < nspin; ++s) eps_prev[s] = eps[s];
     int num_iter = 1, n_fp64 = 0, start_diis = ctrl.start_diis_cycle;
     int nvec = 0, head = 0;   // DIIS ring: vectors [head, head+nvec) mod nd
     std::vector<double> Bmat((size_t)nd * nd, 0.0);
@@ -372,15 +454,22 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
         double diis_max = 0.0;
         const double jk0 = t_jk, dg0 = t_diag;
         const int prec_used = prec;
-        if (num_iter < start_diis) {
-            hipLaunchKernelGGL(k_mix, g2, b256, 0, st, Dp, D, ctrl.mix_param, n2, Dm);
-            SCF_HIP(hipGetLastError());
+        auto linear_mix = [&]() -> int {   // n1[in] = n0[in] + α (n0[out] − n0[in])
+            for (int s = 0; s < nspin; ++s) {
+                hipLaunchKernelGGL(k_mix, g2, b256, 0, st, Dp[s], D[s], ctrl.mix_param, n2, Dm[s]);
+                SCF_HIP(hipGetLastError());
+            }
             SCF_RC(fock(Dm, false, prec, &E));
-            SCF_HIP(hipMemcpyAsync(Fx, F, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+            for (int s = 0; s < nspin; ++s)
+                SCF_HIP(hipMemcpyAsync(Fx[s], F[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+            return RJK_OK;
+        };
+        if (num_iter < start_diis) {
+            SCF_RC(linear_mix());
         } else {
             SCF_RC(fock(D, true, prec, &E));
             const double t0 = now();
-            // error vector e = F D S − S D F into the ring slot
+            // error vector e_s = F_s D_s S − S D_s F_s, spins back to back, into the ring slot
             if (nvec == nd) {   // drop the oldest
                 head = (head + 1) % nd;
                 --nvec;
@@ -388,21 +477,27 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
                     for (int j = 0; j < nd - 1; ++j) Bmat[(size_t)i * nd + j] = Bmat[(size_t)(i + 1) * nd + (j + 1)];
             }
             const int slot = (head + nvec) % nd;
-            double* e_new = ering + (size_t)slot * n2;
-            SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
-                                   (rocblas_int)n, &one, F, (rocblas_int)n, D, (rocblas_int)n, &zero, T1,
-                                   (rocblas_int)n));
-            SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
-                                   (rocblas_int)n, &one, T1, (rocblas_int)n, S, (rocblas_int)n, &zero, T2,
-                                   (rocblas_int)n));
-            hipLaunchKernelGGL(k_commutator, dim3(kRedBlocks), b256, 0, st, T2, n, e_new, partial);
-            SCF_HIP(hipGetLastError());
-            SCF_RC(reduce_max(&diis_max));
-            SCF_HIP(hipMemcpyAsync(fring + (size_t)slot * n2, F, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+            double* e_new = ering + (size_t)slot * ev_len;
+            for (int s = 0; s < nspin; ++s) {
+                SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
+                                       (rocblas_int)n, (rocblas_int)n, &one, F[s], (rocblas_int)n, D[s],
+                                       (rocblas_int)n, &zero, T1, (rocblas_int)n));
+                SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
+                                       (rocblas_int)n, (rocblas_int)n, &one, T1, (rocblas_int)n, S, (rocblas_int)n,
+                                       &zero, T2, (rocblas_int)n));
+                hipLaunchKernelGGL(k_commutator, dim3(kRedBlocks), b256, 0, st, T2, n, e_new + (size_t)s * n2, partial);
+                SCF_HIP(hipGetLastError());
+                double m = 0.0;
+                SCF_RC(reduce_max(&m));
+                diis_max = std::max(diis_max, m);
+                SCF_HIP(hipMemcpyAsync(fring + (size_t)slot * ev_len + (size_t)s * n2, F[s], n2 * sizeof(double),
+                                       hipMemcpyDeviceToDevice, st));
+            }
             ++nvec;
             for (int k = 0; k < nvec; ++k) {
                 double dot = 0.0;
-                SCF_BLAS(rocblas_ddot(h, (rocblas_int)n2, e_new, 1, ering + (size_t)((head + k) % nd) * n2, 1, &dot));
+                SCF_BLAS(rocblas_ddot(h, (rocblas_int)ev_len, e_new, 1, ering + (size_t)((head + k) % nd) * ev_len, 1,
+                                      &dot));
                 Bmat[(size_t)(nvec - 1) * nd + k] = Bmat[(size_t)k * nd + (nvec - 1)] = dot;
             }
             std::vector<double> bm((size_t)nvec * nvec), inv;
@@ -411,20 +506,19 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
             if (small_inverse(bm, nvec, &inv)) {
                 double tot = 0.0;
                 for (double v : inv) tot += v;
-                Coefs cf;
-                for (int k = 0; k < nvec; ++k) {
-                    double r = 0.0;
-                    for (int j = 0; j < nvec; ++j) r += inv[(size_t)k * nvec + j];
-                    cf.c[k] = r / tot;
-                    cf.f[k] = fring + (size_t)((head + k) % nd) * n2;
+                for (int s = 0; s < nspin; ++s) {
+                    Coefs cf;
+                    for (int k = 0; k < nvec; ++k) {
+                        double r = 0.0;
+                        for (int j = 0; j < nvec; ++j) r += inv[(size_t)k * nvec + j];
+                        cf.c[k] = r / tot;
+                        cf.f[k] = fring + (size_t)((head + k) % nd) * ev_len + (size_t)s * n2;
+                    }
+                    hipLaunchKernelGGL(k_lincomb, dim3(kRedBlocks), b256, 0, st, cf, nvec, n2, Fx[s]);
+                    SCF_HIP(hipGetLastError());
                 }
-                hipLaunchKernelGGL(k_lincomb, dim3(kRedBlocks), b256, 0, st, cf, nvec, n2, Fx);
-                SCF_HIP(hipGetLastError());
             } else {   // DIIS failed: linear mixing now, DIIS again 8 cycles later (REST :3061-3078)
-                hipLaunchKernelGGL(k_mix, g2, b256, 0, st, Dp, D, ctrl.mix_param, n2, Dm);
-                SCF_HIP(hipGetLastError());
-                SCF_RC(fock(Dm, false, prec, &E));
-                SCF_HIP(hipMemcpyAsync(Fx, F, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+                SCF_RC(linear_mix());
                 start_diis = num_iter + 8;
                 nvec = 0;
                 head = 0;
@@ -435,28 +529,31 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
         }
         prec_E = prec_used;
         if (prec_used == RJK_PREC_FP64) ++n_fp64;
-        // diagonalise the extrapolated Fock matrix, new density
-        SCF_HIP(hipMemcpyAsync(Dp, D, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        SCF_RC(diagonalise(Fx));
-        SCF_RC(density(D));
-        hipLaunchKernelGGL(k_sumsq_diff, dim3(kRedBlocks), b256, 0, st, D, Dp, n2, partial);
-        SCF_HIP(hipGetLastError());
-        double ss = 0.0;
-        SCF_RC(reduce_sum(&ss));
-        const double dm_err = std::sqrt(ss) / (double)n;
-        double eev = 0.0;
-        for (size_t i = 0; i < n; ++i) eev += (eps[i] - eps_prev[i]) * (eps[i] - eps_prev[i]);
+        // diagonalise the extrapolated Fock matrices, new densities
+        double dm_err[2] = {0.0, 0.0}, eev = 0.0;
+        for (int s = 0; s < nspin; ++s) {
+            SCF_HIP(hipMemcpyAsync(Dp[s], D[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+            SCF_RC(diagonalise(Fx[s], s));
+            SCF_RC(density(s, D[s]));
+            hipLaunchKernelGGL(k_sumsq_diff, dim3(kRedBlocks), b256, 0, st, D[s], Dp[s], n2, partial);
+            SCF_HIP(hipGetLastError());
+            double ss = 0.0;
+            SCF_RC(reduce_sum(&ss));
+            dm_err[s] = std::sqrt(ss) / (double)n;
+            for (size_t i = 0; i < n; ++i) eev += (eps[s][i] - eps_prev[s][i]) * (eps[s][i] - eps_prev[s][i]);
+        }
         eev = std::sqrt(eev);
+        const double dm_max = std::max(dm_err[0], dm_err[1]);
         const double dE = E - E_prev;
         const bool both_fp64 = prec_E == RJK_PREC_FP64 && prec_prev == RJK_PREC_FP64;
-        converged = both_fp64 && std::fabs(dE) <= ctrl.scf_acc_etot && dm_err <= ctrl.scf_acc_rho &&
+        converged = both_fp64 && std::fabs(dE) <= ctrl.scf_acc_etot && dm_max <= ctrl.scf_acc_rho &&
                     eev <= ctrl.scf_acc_eev;
         if (iter_log && num_iter - 1 < ctrl.max_scf_cycle) {
             double* row = iter_log + (size_t)(num_iter - 1) * RSCF_LOG_FIELDS;
             row[RSCF_LOG_E] = E;
             row[RSCF_LOG_DE] = dE;
             row[RSCF_LOG_DIIS_MAX] = diis_max;
-            row[RSCF_LOG_DM] = dm_err;
+            row[RSCF_LOG_DM] = dm_max;
             row[RSCF_LOG_EEV] = eev;
             row[RSCF_LOG_PREC] = prec_used;
             row[RSCF_LOG_JK_MS] = (t_jk - jk0) * 1e3;
@@ -465,10 +562,10 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
         }
         if (ctrl.print_level > 0)
             std::printf("rscf: iter %3d  E %.10f  dE %.3e  diis %.3e  dD %.3e  eev %.3e  prec %s\n", num_iter, E, dE,
-                        diis_max, dm_err, eev, prec_used == RJK_PREC_FP64 ? "fp64" : "fp32");
+                        diis_max, dm_max, eev, prec_used == RJK_PREC_FP64 ? "fp64" : "fp32");
         E_prev = E;
         prec_prev = prec_E;
-        eps_prev = eps;
+        for (int s = 0; s < nspin; ++s) eps_prev[s] = eps[s];
         ++num_iter;
         if (converged || num_iter > ctrl.max_scf_cycle) break;
     }
@@ -483,8 +580,121 @@ extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S_h, const double* H_h, doub
         result->diag_seconds = t_diag;
         result->diis_seconds = t_diis;
     }
-    if (mo_coeff) SCF_HIP(hipMemcpy(mo_coeff, A, n2 * sizeof(double), hipMemcpyDeviceToHost));
-    if (mo_energy) std::memcpy(mo_energy, eps.data(), n * sizeof(double));
-    if (dm_out) SCF_HIP(hipMemcpy(dm_out, D, n2 * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < nspin; ++s) {
+        if (mo_coeff) SCF_HIP(hipMemcpy(mo_coeff + (size_t)s * n2, A[s], n2 * sizeof(double), hipMemcpyDeviceToHost));
+        if (mo_energy) std::memcpy(mo_energy + (size_t)s * n, eps[s].data(), n * sizeof(double));
+        if (dm_out) SCF_HIP(hipMemcpy(dm_out + (size_t)s * n2, D[s], n2 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return RJK_OK;
+}
+
+}  // namespace
+
+extern "C" int rscf_rhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nelec,
+                        const double* d_guess, const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff,
+                        double* mo_energy, double* dm, double* iter_log) {
+    if (!ctx) return RJK_ERR_ARG;
+    const size_t nel[2] = {nelec, 0};
+    return scf_core(ctx, "rscf_rhf", S, H, enuc, 1, nel, d_guess, ctrl, false, result, mo_coeff, mo_energy, dm,
+                    iter_log);
+}
+
+extern "C" int rscf_uhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nalpha, size_t nbeta,
+                        const double* d_guess, const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff,
+                        double* mo_energy, double* dm, double* iter_log) {
+    if (!ctx) return RJK_ERR_ARG;
+    const size_t nel[2] = {nalpha, nbeta};
+    return scf_core(ctx, "rscf_uhf", S, H, enuc, 2, nel, d_guess, ctrl, false, result, mo_coeff, mo_energy, dm,
+                    iter_log);
+}
+
+extern "C" int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin,
+                               const size_t* nelec, const double* d0, int prec, double* energy, double* d1,
+                               double* mo_coeff, double* mo_energy) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!nelec || !d0 || (nspin != 1 && nspin != 2))
+        return rjk::fail(ctx, RJK_ERR_ARG, "rscf_guess_step: nspin must be 1 or 2, nelec and d0 non-NULL");
+    const size_t nel[2] = {nelec[0], nspin == 2 ? nelec[1] : 0};
+    rscf_ctrl c;
+    rscf_default_ctrl(&c);
+    c.jk_prec = prec;
+    rscf_result r{};
+    const int rc = scf_core(ctx, "rscf_guess_step", S, H, enuc, (int)nspin, nel, d0, &c, true, &r, mo_coeff,
+                            mo_energy, d1, nullptr);
+    if (!rc && energy) *energy = r.energy;
+    return rc;
+}
+
+namespace {
+// out = a·X + b·Y
+__global__ void k_axpby(const double* __restrict__ X, const double* __restrict__ Y, double a, double b, size_t n2,
+                        double* __restrict__ out) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n2) out[t] = a * X[t] + b * Y[t];
+}
+// partial[b] = Σ_i M[i][i]
+__global__ void k_trace(const double* __restrict__ M, size_t n, double* __restrict__ partial) {
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc += M[i * n + i];
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+}  // namespace
+
+extern "C" int rscf_mcweeny(rjk_ctx* ctx, const double* S_h, const double* D_h, int iterations, double* D_out,
+                            double* electrons) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!S_h || !D_h || !D_out || iterations < 0 || iterations > 64)
+        return rjk::fail(ctx, RJK_ERR_ARG, "rscf_mcweeny: S, D, D_out required and 0 <= iterations <= 64");
+    const size_t n = ctx->nbf, n2 = n * n;
+    SCF_HIP(hipSetDevice(ctx->device));
+    const hipStream_t st = ctx->stream;
+    BlasHandle bh;
+    SCF_BLAS(rocblas_create_handle(&bh.h));
+    SCF_BLAS(rocblas_set_stream(bh.h, st));
+    int rc = RJK_OK;
+    DevMem mem;
+    double* S = mem.get(ctx, n2, &rc);
+    double* P = mem.get(ctx, n2, &rc);
+    double* T1 = mem.get(ctx, n2, &rc);
+    double* T2 = mem.get(ctx, n2, &rc);
+    double* T3 = mem.get(ctx, n2, &rc);
+    double* partial = mem.get(ctx, kRedBlocks, &rc);
+    if (rc) return rc;
+    SCF_HIP(hipMemcpyAsync(S, S_h, n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    SCF_HIP(hipMemcpyAsync(T1, D_h, n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    const dim3 g2((unsigned)((n2 + 255) / 256)), b256(256);
+    hipLaunchKernelGGL(k_axpby, g2, b256, 0, st, T1, T1, 0.5, 0.0, n2, P);   // P = D / 2
+    SCF_HIP(hipGetLastError());
+    const double one = 1.0, zero = 0.0;
+    const rocblas_int ni = (rocblas_int)n;
+    auto gemm = [&](const double* A, const double* B, double* C) -> int {
+        SCF_BLAS(rocblas_dgemm(bh.h, rocblas_operation_none, rocblas_operation_none, ni, ni, ni, &one, A, ni, B, ni,
+                               &zero, C, ni));
+        return RJK_OK;
+    };
+    for (int it = 0; it < iterations; ++it) {   // P <- 3 PSP − 2 PSPSP
+        SCF_RC(gemm(P, S, T1));
+        SCF_RC(gemm(T1, P, T2));
+        SCF_RC(gemm(T1, T2, T3));
+        hipLaunchKernelGGL(k_axpby, g2, b256, 0, st, T2, T3, 3.0, -2.0, n2, P);
+        SCF_HIP(hipGetLastError());
+    }
+    if (electrons) {   // 2 tr(P S)
+        SCF_RC(gemm(P, S, T1));
+        hipLaunchKernelGGL(k_trace, dim3(kRedBlocks), b256, 0, st, T1, n, partial);
+        SCF_HIP(hipGetLastError());
+        std::vector<double> part(kRedBlocks);
+        SCF_HIP(hipMemcpyAsync(part.data(), partial, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, st));
+        SCF_HIP(hipStreamSynchronize(st));
+        double tr = 0.0;
+        for (double v : part) tr += v;
+        *electrons = 2.0 * tr;
+    }
+    hipLaunchKernelGGL(k_axpby, g2, b256, 0, st, P, P, 2.0, 0.0, n2, T1);   // D = 2 P
+    SCF_HIP(hipGetLastError());
+    SCF_HIP(hipMemcpyAsync(D_out, T1, n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    SCF_HIP(hipStreamSynchronize(st));
     return RJK_OK;
 }

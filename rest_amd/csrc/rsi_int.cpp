@@ -415,6 +415,16 @@ using namespace rsi;
 
 extern "C" const char* rsi_last_error(void) { return last_error().c_str(); }
 
+extern "C" int rsi_cart2sph(int l, double* out) {
+    if (l < 0 || l > kMaxL || !out) {
+        set_error("rsi_cart2sph: l must be 0..4 and out non-NULL");
+        return RJK_ERR_ARG;
+    }
+    const std::vector<double> T = cart2sph(l);
+    std::copy(T.begin(), T.end(), out);
+    return RJK_OK;
+}
+
 extern "C" int rsi_basis_create(size_t nshell, const int* shell_atom, const int* shell_l, const int* shell_nprim,
                                 const double* exps, const double* coefs, size_t natm, const double* atom_xyz,
                                 rsi_basis** out) {

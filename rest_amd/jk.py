@@ -47,9 +47,12 @@ EXPORTS = [
     "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_profile_enable", "rjk_profile_read", "rjk_copy_dev",
     # include/rest_ri.h
     "rsi_basis_create", "rsi_basis_destroy", "rsi_basis_nbf", "rsi_last_error", "rsi_int1e", "rsi_int2c2e",
-    "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr",
+    "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr", "rsi_cart2sph",
     # include/rest_scf.h
-    "rscf_default_ctrl", "rscf_rhf",
+    "rscf_default_ctrl", "rscf_rhf", "rscf_uhf", "rscf_guess_step", "rscf_mcweeny",
+    # include/rest_lg.h
+    "rlg_create", "rlg_destroy", "rlg_last_error", "rlg_num_params", "rlg_set_stream", "rlg_set_params",
+    "rlg_get_params", "rlg_forward", "rlg_loss_grad", "rlg_adam_step", "rlg_reset_optimizer", "rlg_gemm_bf16",
 ]
 
 
@@ -114,7 +117,20 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rsi_int2c2e": (i, [vp, vp]),
         "rsi_int3c2e_packed": (i, [vp, vp, sz, sz, vp]),
         "rsi_int3c2e_packed_dev": (i, [vp, vp, vp, sz, sz, vp]),
+        "rsi_cart2sph": (i, [i, vp]),
         "rsi_build_rimatr": (i, [vp, vp, vp, d, C.c_uint, C.POINTER(sz), pd]),
+        "rlg_create": (i, [i, sz, sz, sz, sz, sz, C.POINTER(vp)]),
+        "rlg_destroy": (i, [vp]),
+        "rlg_last_error": (C.c_char_p, [vp]),
+        "rlg_num_params": (sz, [vp]),
+        "rlg_set_stream": (i, [vp, vp]),
+        "rlg_set_params": (i, [vp, vp]),
+        "rlg_get_params": (i, [vp, vp]),
+        "rlg_forward": (i, [vp, vp, sz, vp]),
+        "rlg_loss_grad": (i, [vp, vp, vp, vp, sz, pd, vp]),
+        "rlg_adam_step": (i, [vp, d, d, d, d]),
+        "rlg_reset_optimizer": (i, [vp]),
+        "rlg_gemm_bf16": (i, [vp, vp, C.c_long, C.c_long, vp, C.c_long, C.c_long, vp, C.c_long, i, i, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,511 @@
+"""Learned initial guess: a per-atom-pair MLP that predicts density-matrix
+blocks (a new `initial_guess` branch beside REST's "sad",
+src/initial_guess/mod.rs:15-72).
+
+Representation
+  - Pairs: every atom pair (A, B), A < B, within R_CUT bohr, and every atom
+    with itself.  For each, a local frame (rows e_x, e_y, e_z):
+      A ≠ B: e_z along B − A; e_x from the environment vector of A
+             v_A = Σ_C w(|C − A|)(C − A) over atoms C ∉ {A, B} (B's if A's is
+             parallel to e_z), w(r) = exp(−r²/σ_ref²);
+      A = B: e_z along v_A (all neighbours), e_x from the nearest neighbour.
+  - Inputs (N_IN numbers): r_AB/10 and 1/r_AB (0 for A = B), the ordered
+    element pair one-hot over ELEMENTS² and a same-atom flag, 16-bin Gaussian
+    radial neighbour features of A and of B per neighbour element, and v_A,
+    v_B in the pair frame with their lengths.
+  - Outputs: the pair's block of D − D_SAD (the converged closed-shell density
+    minus the superposition of atomic densities) in the pair frame, on a
+    MAXB × MAXB grid (rows: A's functions, columns: B's) with a mask for the
+    element pair's real block size, scaled by TARGET_SCALE.
+  - Blocks are carried between the molecule frame and the pair frame by the
+    real-solid-harmonic rotation matrices of the engine's own convention
+    (rsi_cart2sph): D_loc = U_A D U_Bᵀ.
+
+Guess: D = D_SAD + predicted blocks (pairs beyond R_CUT keep D_SAD's zero),
+symmetrised, McWeeny-purified (rscf_mcweeny), and accepted only if its guess
+energy E(D) (rscf_guess_step) is below that of the SAD density; otherwise the
+SAD guess is used.
+
+The network runs on the device through include/rest_lg.h (BF16 matrix-core
+GEMMs, FP32 master weights, Adam); torch only provides device buffers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import time
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import guess as gs
+from . import jk, ri
+from . import scf as rs
+from . import workload as wl
+
+ELEMENTS = ("H", "C", "N", "O")
+NBINS = 16
+MU = 1.0 + 0.6 * np.arange(NBINS)       # radial bin centres (bohr)
+SIGMA_BIN = 0.6
+R_CUT = 10.0                            # pair cut-off (bohr)
+R_ENV = 12.0                            # neighbour cut-off of the radial features (bohr)
+SIGMA_REF = 2.0                         # width of the environment-vector weights (bohr)
+MAXB = 14                               # largest per-atom block (O, C, N in def2-SVP)
+TARGET_SCALE = 4.0
+N_IN = 2 + len(ELEMENTS) ** 2 + 1 + 2 * len(ELEMENTS) * NBINS + 8
+N_OUT = MAXB * MAXB
+MODEL_DIR = Path(__file__).resolve().parent / "models"
+
+
+# ---------------------------------------------------------------------------
+# rotations of real solid harmonics in the engine's convention
+# ---------------------------------------------------------------------------
+def _cart_exps(l):
+    return [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
+
+
+def _sphere_moment(a, b, c):
+    if a % 2 or b % 2 or c % 2:
+        return 0.0
+    g = math.gamma
+    return 2.0 * g((a + 1) / 2) * g((b + 1) / 2) * g((c + 1) / 2) / g((a + b + c + 3) / 2)
+
+
+_ROWS: dict = {}
+
+
+def harmonic_rows(l: int) -> np.ndarray:
+    """Engine rows (rsi_cart2sph) of shell l, each scaled to unit norm on the unit sphere."""
+    if l in _ROWS:
+        return _ROWS[l]
+    nc = (l + 1) * (l + 2) // 2
+    T = np.empty((2 * l + 1) * nc)
+    rc = jk.load_library().rsi_cart2sph(int(l), T.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"rsi_cart2sph({l}) failed")
+    T = T.reshape(2 * l + 1, nc)
+    ex = _cart_exps(l)
+    G = np.array([[_sphere_moment(*(np.add(e1, e2))) for e2 in ex] for e1 in ex])
+    T = T / np.sqrt(np.einsum("mi,ij,mj->m", T, G, T))[:, None]
+    _ROWS[l] = T
+    return T
+
+
+_PTS = None
+
+
+def _sample_points():
+    global _PTS
+    if _PTS is None:
+        rng = np.random.default_rng(12345)
+        p = rng.standard_normal((48, 3))
+        _PTS = p / np.linalg.norm(p, axis=1, keepdims=True)
+    return _PTS
+
+
+def _eval_rows(l, P):
+    ex = _cart_exps(l)
+    mono = np.stack([P[:, 0] ** a * P[:, 1] ** b * P[:, 2] ** c for a, b, c in ex], axis=1)
+    return mono @ harmonic_rows(l).T
+
+
+def sph_rotation(l: int, R: np.ndarray) -> np.ndarray:
+    """D^l(R): for the molecule rotated by R (r → R r), shell-l coefficients transform as c' = D^l c."""
+    if l == 0:
+        return np.ones((1, 1))
+    P = _sample_points()
+    Phi = _eval_rows(l, P)
+    PhiR = _eval_rows(l, P @ R)          # Y_m(Rᵀ p)
+    return np.linalg.lstsq(Phi, PhiR, rcond=None)[0]
+
+
+def atom_rotation(shell_ls, R: np.ndarray) -> np.ndarray:
+    """Block-diagonal rotation of one atom's functions (its shells in basis order)."""
+    blocks = [sph_rotation(l, R) for l in shell_ls]
+    n = sum(b.shape[0] for b in blocks)
+    U = np.zeros((n, n))
+    o = 0
+    for b in blocks:
+        k = b.shape[0]
+        U[o:o + k, o:o + k] = b
+        o += k
+    return U
+
+
+def molecule_rotation(mol: "ri.Molecule", R: np.ndarray) -> np.ndarray:
+    """U with D(rotated molecule) = U D Uᵀ for the whole orbital basis."""
+    n = mol.nbf
+    U = np.zeros((n, n))
+    for a, (o, ls) in enumerate(atom_layout(mol)):
+        Ua = atom_rotation(ls, R)
+        U[o:o + Ua.shape[0], o:o + Ua.shape[0]] = Ua
+    return U
+
+
+def atom_layout(mol: "ri.Molecule") -> list:
+    """[(first function, [shell l ...]) per atom] of the orbital basis."""
+    b = mol.orb
+    out = []
+    for a in range(len(mol.symbols)):
+        idx = np.nonzero(b.shell_atom == a)[0]
+        out.append((int(b.shell_off[idx[0]]), [int(l) for l in b.shell_l[idx]]))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# pairs, frames and descriptors
+# ---------------------------------------------------------------------------
+def _unit(v):
+    n = np.linalg.norm(v)
+    return v / n if n > 0 else v
+
+
+def _perp(v, z):
+    return v - np.dot(v, z) * z
+
+
+def pair_list(xyz: np.ndarray, cutoff: float = R_CUT) -> np.ndarray:
+    """(A, B) with A <= B and |B − A| <= cutoff, diagonal included, in row-major order."""
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=-1)
+    A, B = np.nonzero(np.triu(d <= cutoff))
+    return np.stack([A, B], axis=1)
+
+
+def _env_vector(xyz, a, exclude):
+    d = xyz - xyz[a]
+    r2 = np.sum(d * d, axis=1)
+    w = np.exp(-r2 / SIGMA_REF ** 2)
+    w[list(exclude)] = 0.0
+    return (w[:, None] * d).sum(axis=0)
+
+
+def pair_frame(xyz: np.ndarray, a: int, b: int) -> np.ndarray:
+    """Rows e_x, e_y, e_z of the local frame of pair (a, b)."""
+    if a != b:
+        z = _unit(xyz[b] - xyz[a])
+        cands = [_env_vector(xyz, a, (a, b)), _env_vector(xyz, b, (a, b)), np.array([1.0, 0.0, 0.0]),
+                 np.array([0.0, 1.0, 0.0])]
+    else:
+        v = _env_vector(xyz, a, (a,))
+        d = np.linalg.norm(xyz - xyz[a], axis=1)
+        d[a] = np.inf
+        order = np.argsort(d, kind="stable")
+        z = _unit(v) if np.linalg.norm(v) > 1e-8 else (_unit(xyz[order[0]] - xyz[a]) if len(xyz) > 1
+                                                        else np.array([0.0, 0.0, 1.0]))
+        cands = [xyz[c] - xyz[a] for c in order[:3] if np.isfinite(d[c])] + [np.array([1.0, 0.0, 0.0]),
+                                                                              np.array([0.0, 1.0, 0.0])]
+    for v in cands:
+        p = _perp(v, z)
+        if np.linalg.norm(p) > 1e-3 * max(1.0, np.linalg.norm(v)):
+            x = _unit(p)
+            return np.stack([x, np.cross(z, x), z])
+    raise RuntimeError("no local frame")
+
+
+def radial_features(symbols, xyz: np.ndarray) -> np.ndarray:
+    """[natom, len(ELEMENTS) * NBINS] Gaussian radial neighbour histograms."""
+    n = len(symbols)
+    el = np.array([ELEMENTS.index(s) for s in symbols])
+    d = np.linalg.norm(xyz[:, None, :] - xyz[None, :, :], axis=-1)
+    g = np.exp(-((d[:, :, None] - MU[None, None, :]) / SIGMA_BIN) ** 2)
+    g[(d > R_ENV) | (d == 0.0)] = 0.0
+    out = np.zeros((n, len(ELEMENTS), NBINS))
+    for e in range(len(ELEMENTS)):
+        out[:, e, :] = g[:, el == e, :].sum(axis=1)
+    return out.reshape(n, -1)
+
+
+def descriptors(symbols, xyz: np.ndarray, pairs: np.ndarray):
+    """(features [npair, N_IN] float32, frames [npair, 3, 3])."""
+    rad = radial_features(symbols, xyz)
+    el = [ELEMENTS.index(s) for s in symbols]
+    ne = len(ELEMENTS)
+    X = np.zeros((len(pairs), N_IN), dtype=np.float64)
+    F = np.empty((len(pairs), 3, 3))
+    for k, (a, b) in enumerate(pairs):
+        R = pair_frame(xyz, a, b)
+        F[k] = R
+        r = np.linalg.norm(xyz[b] - xyz[a])
+        o = 0
+        X[k, 0] = r / 10.0
+        X[k, 1] = 1.0 / r if a != b else 0.0
+        o = 2
+        X[k, o + el[a] * ne + el[b]] = 1.0
+        o += ne * ne
+        X[k, o] = 1.0 if a == b else 0.0
+        o += 1
+        X[k, o:o + rad.shape[1]] = rad[a]
+        o += rad.shape[1]
+        X[k, o:o + rad.shape[1]] = rad[b]
+        o += rad.shape[1]
+        va = R @ _env_vector(xyz, a, (a, b))
+        vb = R @ _env_vector(xyz, b, (a, b))
+        X[k, o:o + 3] = va
+        X[k, o + 3] = np.linalg.norm(va)
+        X[k, o + 4:o + 7] = vb
+        X[k, o + 7] = np.linalg.norm(vb)
+    return X.astype(np.float32), F
+
+
+# ---------------------------------------------------------------------------
+# encoding densities as per-pair targets and back
+# ---------------------------------------------------------------------------
+def encode(mol: "ri.Molecule", D: np.ndarray, D_sad: np.ndarray):
+    """Training arrays (X, T, M) of one molecule with converged density D."""
+    pairs = pair_list(mol.xyz_bohr)
+    X, F = descriptors(mol.symbols, mol.xyz_bohr, pairs)
+    lay = atom_layout(mol)
+    T = np.zeros((len(pairs), MAXB, MAXB), dtype=np.float32)
+    M = np.zeros_like(T)
+    dD = D - D_sad
+    for k, (a, b) in enumerate(pairs):
+        (oa, la), (ob, lb) = lay[a], lay[b]
+        Ua, Ub = atom_rotation(la, F[k]), atom_rotation(lb, F[k])
+        na, nb = Ua.shape[0], Ub.shape[0]
+        if na > MAXB or nb > MAXB:
+            raise ValueError(f"atom blocks above {MAXB} functions are not supported")
+        T[k, :na, :nb] = TARGET_SCALE * (Ua @ dD[oa:oa + na, ob:ob + nb] @ Ub.T)
+        M[k, :na, :nb] = 1.0
+    return X, T.reshape(len(pairs), -1), M.reshape(len(pairs), -1)
+
+
+def decode(mol: "ri.Molecule", pairs: np.ndarray, frames: np.ndarray, Y: np.ndarray, D_sad: np.ndarray):
+    """D = D_SAD + Σ predicted pair blocks rotated back, symmetric."""
+    lay = atom_layout(mol)
+    dD = np.zeros_like(D_sad)
+    Y = Y.reshape(len(pairs), MAXB, MAXB) / TARGET_SCALE
+    for k, (a, b) in enumerate(pairs):
+        (oa, la), (ob, lb) = lay[a], lay[b]
+        Ua, Ub = atom_rotation(la, frames[k]), atom_rotation(lb, frames[k])
+        na, nb = Ua.shape[0], Ub.shape[0]
+        blk = Ua.T @ Y[k, :na, :nb].astype(np.float64) @ Ub
+        if a == b:
+            dD[oa:oa + na, oa:oa + na] += 0.5 * (blk + blk.T)
+        else:
+            dD[oa:oa + na, ob:ob + nb] += blk
+            dD[ob:ob + nb, oa:oa + na] += blk.T
+    return D_sad + dD
+
+
+# ---------------------------------------------------------------------------
+# the device network
+# ---------------------------------------------------------------------------
+class Net:
+    """ctypes owner of an rlg_net (include/rest_lg.h)."""
+
+    def __init__(self, n_in: int = N_IN, n_out: int = N_OUT, hidden: int = 256, layers: int = 3,
+                 max_batch: int = 8192, device: int = 0, seed: int = 0):
+        self.lib = jk.load_library()
+        self.dims = [n_in] + [hidden] * layers + [n_out]
+        self.max_batch = max_batch
+        h = C.c_void_p()
+        rc = self.lib.rlg_create(int(device), n_in, hidden, layers, n_out, max_batch, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"rlg_create failed ({rc})")
+        self.h = h
+        self.nparam = int(self.lib.rlg_num_params(h))
+        self.set_params(self.init_params(seed))
+        self.input_mean = np.zeros(n_in, dtype=np.float32)
+        self.input_std = np.ones(n_in, dtype=np.float32)
+
+    def init_params(self, seed: int) -> np.ndarray:
+        """He-uniform weights for the SiLU layers, zero biases."""
+        rng = np.random.default_rng(seed)
+        out = []
+        for l, (din, dout) in enumerate(zip(self.dims[:-1], self.dims[1:])):
+            lim = math.sqrt(6.0 / din) if l + 1 < len(self.dims) - 1 else math.sqrt(3.0 / din) * 0.1
+            out.append(rng.uniform(-lim, lim, size=din * dout))
+            out.append(np.zeros(dout))
+        return np.concatenate(out).astype(np.float32)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: {self.lib.rlg_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rlg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self._check(self.lib.rlg_set_params(self.h, p.ctypes.data), "rlg_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.nparam, dtype=np.float32)
+        self._check(self.lib.rlg_get_params(self.h, p.ctypes.data), "rlg_get_params")
+        return p
+
+    def set_stream(self, stream_ptr: int):
+        self._check(self.lib.rlg_set_stream(self.h, C.c_void_p(stream_ptr)), "rlg_set_stream")
+
+    def normalise(self, X: np.ndarray) -> np.ndarray:
+        return ((X - self.input_mean) / self.input_std).astype(np.float32)
+
+    def forward_dev(self, x, y):
+        """x, y: contiguous float32 torch CUDA tensors [batch, dims]."""
+        self._check(self.lib.rlg_forward(self.h, C.c_void_p(x.data_ptr()), x.shape[0], C.c_void_p(y.data_ptr())),
+                    "rlg_forward")
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        """Host features (not yet normalised) -> host outputs."""
+        import torch
+        Xn = self.normalise(X)
+        out = np.empty((len(Xn), self.dims[-1]), dtype=np.float32)
+        for i0 in range(0, len(Xn), self.max_batch):
+            x = torch.from_numpy(Xn[i0:i0 + self.max_batch]).cuda()
+            y = torch.empty((x.shape[0], self.dims[-1]), dtype=torch.float32, device="cuda")
+            self.forward_dev(x, y)
+            out[i0:i0 + len(x)] = y.cpu().numpy()
+        return out
+
+    def loss_grad_dev(self, x, t, m, want_grad: bool = False):
+        loss = C.c_double(0.0)
+        g = np.empty(self.nparam, dtype=np.float32) if want_grad else None
+        self._check(self.lib.rlg_loss_grad(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                           C.c_void_p(m.data_ptr()), x.shape[0], C.byref(loss),
+                                           None if g is None else g.ctypes.data), "rlg_loss_grad")
+        return loss.value, g
+
+    def adam(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+        self._check(self.lib.rlg_adam_step(self.h, lr, b1, b2, eps), "rlg_adam_step")
+
+    def reset_optimizer(self):
+        self._check(self.lib.rlg_reset_optimizer(self.h), "rlg_reset_optimizer")
+
+    def save(self, path: Path, meta: Optional[dict] = None):
+        np.savez(path, params=self.get_params(), dims=np.array(self.dims), input_mean=self.input_mean,
+                 input_std=self.input_std, meta=json.dumps(meta or {}))
+
+    @classmethod
+    def load(cls, path: Path, max_batch: int = 8192, device: int = 0) -> "Net":
+        z = np.load(path, allow_pickle=False)
+        dims = [int(d) for d in z["dims"]]
+        net = cls(dims[0], dims[-1], dims[1], len(dims) - 2, max_batch, device)
+        net.set_params(z["params"])
+        net.input_mean = z["input_mean"].astype(np.float32)
+        net.input_std = z["input_std"].astype(np.float32)
+        net.meta = json.loads(str(z["meta"]))
+        return net
+
+
+def train(net: Net, X: np.ndarray, T: np.ndarray, M: np.ndarray, epochs: int = 20, batch: int = 4096,
+          lr: float = 1e-3, lr_min: float = 3e-5, seed: int = 0, val_fraction: float = 0.05, log=print) -> dict:
+    """Masked-MSE training with Adam and a cosine learning-rate schedule, all on the device."""
+    import torch
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(X))
+    nval = max(1, int(len(X) * val_fraction))
+    net.input_mean = X.mean(axis=0).astype(np.float32)
+    net.input_std = (X.std(axis=0) + 1e-3).astype(np.float32)
+    Xn = net.normalise(X)[perm]
+    x_all = torch.from_numpy(Xn).cuda()
+    t_all = torch.from_numpy(np.ascontiguousarray(T[perm])).cuda()
+    m_all = torch.from_numpy(np.ascontiguousarray(M[perm])).cuda()
+    ntr = len(X) - nval
+    nb = max(1, ntr // batch)
+    steps = epochs * nb
+    hist = {"train": [], "val": []}
+    step = 0
+    t0 = time.time()
+    for ep in range(epochs):
+        tot = 0.0
+        for bi in rng.permutation(nb):
+            i0 = nval + bi * batch
+            i1 = min(nval + ntr, i0 + batch)
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / max(1, steps - 1)))
+            loss, _ = net.loss_grad_dev(x_all[i0:i1], t_all[i0:i1], m_all[i0:i1])
+            net.adam(cur)
+            tot += loss
+            step += 1
+        vl = 0.0
+        for i0 in range(0, nval, batch):
+            i1 = min(nval, i0 + batch)
+            vl += net.loss_grad_dev(x_all[i0:i1], t_all[i0:i1], m_all[i0:i1])[0] * (i1 - i0)
+        hist["train"].append(tot / nb)
+        hist["val"].append(vl / nval)
+        log(f"epoch {ep + 1:3d}/{epochs}  train {tot / nb:.3e}  val {vl / nval:.3e}  ({time.time() - t0:.1f}s)")
+    return hist
+
+
+# ---------------------------------------------------------------------------
+# the guess
+# ---------------------------------------------------------------------------
+def predict_density(net: Net, mol: "ri.Molecule", D_sad: np.ndarray) -> np.ndarray:
+    pairs = pair_list(mol.xyz_bohr)
+    X, F = descriptors(mol.symbols, mol.xyz_bohr, pairs)
+    Y = net.predict(X)
+    return decode(mol, pairs, F, Y, D_sad)
+
+
+@dataclass
+class LearnedGuess:
+    d1: np.ndarray              # density the SCF starts from
+    kind: str                   # "learned" or "sad"
+    e_learned: float
+    e_sad: float
+    electrons: float            # 2 tr(PS) of the purified prediction
+    seconds: dict = field(default_factory=dict)
+
+
+def learned_guess(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, prec: int = jk.PREC_FP64,
+                  mcweeny_iterations: int = 3) -> LearnedGuess:
+    """Predict, purify, and keep the prediction only if its guess energy is below SAD's."""
+    t0 = time.perf_counter()
+    da, db = gs.sad_density(mol)
+    d_sad = da + db
+    t1 = time.perf_counter()
+    d_pred = predict_density(net, mol, d_sad)
+    t2 = time.perf_counter()
+    d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
+    t3 = time.perf_counter()
+    enuc = mol.nuclear_repulsion()
+    nel = (mol.nelectron,)
+    e_l, d1_l, _, _ = rs.guess_step(r, S, H, enuc, nel, d_pur, prec)
+    e_s, d1_s, _, _ = rs.guess_step(r, S, H, enuc, nel, d_sad, prec)
+    t4 = time.perf_counter()
+    ok = np.isfinite(e_l) and abs(ne - mol.nelectron) < 0.5 and e_l < e_s
+    secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "guess_steps": t4 - t3}
+    return LearnedGuess(d1_l if ok else d1_s, "learned" if ok else "sad", e_l, e_s, ne, secs)
+
+
+# ---------------------------------------------------------------------------
+# training data from this SCF
+# ---------------------------------------------------------------------------
+def training_cluster(i: int, base_seed: int = 4096, nmin: int = 2, nmax: int = 24):
+    """Cluster i of the training set: n uniform in [nmin, nmax], monomers with O–H ±0.05 Å and H–O–H ±5°."""
+    r = wl.Xoshiro256((base_seed ^ (i * 0x9E3779B97F4A7C15)) & ((1 << 64) - 1))
+    n = nmin + int(r.uniform() * (nmax - nmin + 1))
+    seed = r.next()
+    return wl.water_cluster(seed, n, oh_jitter=0.05, hoh_jitter_deg=5.0)
+
+
+def make_training_data(indices, base_seed: int = 4096, basis="def2-SVP", auxbasis="def2-SVP-JKFIT", log=print):
+    """Run the FP64 SCF (SAD start) on each training cluster and encode its converged density."""
+    Xs, Ts, Ms = [], [], []
+    t0 = time.time()
+    for c, i in enumerate(indices):
+        xyz = training_cluster(i, base_seed)
+        mol = ri.Molecule.water_cluster(xyz, basis, auxbasis)
+        out, _ = rs.run_molecule(mol, guess="sad")
+        if not out.converged:
+            log(f"cluster {i}: SCF did not converge, skipped")
+            continue
+        da, db = gs.sad_density(mol)
+        X, T, M = encode(mol, out.dm, da + db)
+        Xs.append(X)
+        Ts.append(T)
+        Ms.append(M)
+        if (c + 1) % 16 == 0:
+            log(f"  {c + 1}/{len(indices)} clusters, {sum(len(x) for x in Xs)} pairs ({time.time() - t0:.0f}s)")
+    return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms)

@@ -39,6 +39,14 @@ def _bind(lib):
     lib.rscf_rhf.restype = C.c_int
     lib.rscf_rhf.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, C.POINTER(Ctrl), C.POINTER(Result), vp, vp,
                              vp, vp]
+    lib.rscf_uhf.restype = C.c_int
+    lib.rscf_uhf.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, C.c_size_t, vp, C.POINTER(Ctrl),
+                             C.POINTER(Result), vp, vp, vp, vp]
+    lib.rscf_guess_step.restype = C.c_int
+    lib.rscf_guess_step.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, C.c_int, C.POINTER(C.c_double), vp,
+                                    vp, vp]
+    lib.rscf_mcweeny.restype = C.c_int
+    lib.rscf_mcweeny.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_double)]
     lib._rscf_bound = True
     return lib
 
@@ -68,35 +76,94 @@ class ScfOutcome:
     log: list = field(default_factory=list)
 
 
-def rhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nelec: int,
-        d_guess: Optional[np.ndarray] = None, ctrl: Optional[Ctrl] = None, want_orbitals: bool = True) -> ScfOutcome:
+def _colmajor(M):
+    return np.ascontiguousarray(np.asarray(M, dtype=np.float64).T)
+
+
+def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals) -> ScfOutcome:
     lib = _bind(jk.load_library())
     n = ri.nbf
+    ns = len(nel)
     ctrl = ctrl or default_ctrl()
-    Sf = np.ascontiguousarray(np.asarray(S, dtype=np.float64).T)   # column-major
-    Hf = np.ascontiguousarray(np.asarray(H, dtype=np.float64).T)
-    dg = None if d_guess is None else np.ascontiguousarray(np.asarray(d_guess, dtype=np.float64).T)
+    Sf, Hf = _colmajor(S), _colmajor(H)
+    dg = None
+    if d_guess is not None:
+        dgs = [d_guess] if ns == 1 else list(d_guess)
+        dg = np.ascontiguousarray(np.stack([_colmajor(d) for d in dgs]))
     res = Result()
-    C_ = np.empty((n, n)) if want_orbitals else None
-    eps = np.empty(n)
-    dm = np.empty((n, n)) if want_orbitals else None
+    C_ = np.empty((ns, n, n)) if want_orbitals else None
+    eps = np.empty((ns, n))
+    dm = np.empty((ns, n, n)) if want_orbitals else None
     log = np.zeros((ctrl.max_scf_cycle, len(LOG_FIELDS)))
-    rc = lib.rscf_rhf(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), int(nelec),
-                      None if dg is None else dg.ctypes.data, C.byref(ctrl), C.byref(res),
-                      None if C_ is None else C_.ctypes.data, eps.ctypes.data, None if dm is None else dm.ctypes.data,
-                      log.ctypes.data)
+    ptr = lambda a: None if a is None else a.ctypes.data
+    if ns == 1:
+        rc = lib.rscf_rhf(ri.h, ptr(Sf), ptr(Hf), float(enuc), int(nel[0]), ptr(dg), C.byref(ctrl), C.byref(res),
+                          ptr(C_), ptr(eps), ptr(dm), ptr(log))
+    else:
+        rc = lib.rscf_uhf(ri.h, ptr(Sf), ptr(Hf), float(enuc), int(nel[0]), int(nel[1]), ptr(dg), C.byref(ctrl),
+                          C.byref(res), ptr(C_), ptr(eps), ptr(dm), ptr(log))
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
     rows = [dict(zip(LOG_FIELDS, r)) for r in log[:res.iterations]]
+    tr = (lambda a: None if a is None else (a[0].T.copy() if ns == 1 else np.stack([x.T.copy() for x in a])))
     return ScfOutcome(res.energy, bool(res.converged), res.iterations, res.fp64_iterations, res.wall_seconds,
-                      res.jk_seconds, res.diag_seconds, res.diis_seconds, eps,
-                      None if C_ is None else C_.T.copy(), None if dm is None else dm.T.copy(), rows)
+                      res.jk_seconds, res.diag_seconds, res.diis_seconds, eps[0] if ns == 1 else eps, tr(C_), tr(dm),
+                      rows)
+
+
+def rhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nelec: int,
+        d_guess: Optional[np.ndarray] = None, ctrl: Optional[Ctrl] = None, want_orbitals: bool = True) -> ScfOutcome:
+    """Closed-shell RI-HF on ri's resident B (rscf_rhf)."""
+    return _run(ri, S, H, enuc, (nelec,), d_guess, ctrl, want_orbitals)
+
+
+def uhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nalpha: int, nbeta: int,
+        d_guess=None, ctrl: Optional[Ctrl] = None, want_orbitals: bool = True) -> ScfOutcome:
+    """Unrestricted RI-HF (rscf_uhf); d_guess is (D_α, D_β) or None; orbitals/densities are [2, n, n]."""
+    return _run(ri, S, H, enuc, (nalpha, nbeta), d_guess, ctrl, want_orbitals)
+
+
+def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PREC_FP64):
+    """REST's density-guess step (rscf_guess_step): returns (E(D0), D1, C, ε); nel = (nelec,) or (nα, nβ)."""
+    lib = _bind(jk.load_library())
+    n = ri.nbf
+    ns = len(nel)
+    d0s = [d0] if ns == 1 else list(d0)
+    d0f = np.ascontiguousarray(np.stack([_colmajor(d) for d in d0s]))
+    d1 = np.empty((ns, n, n))
+    C_ = np.empty((ns, n, n))
+    eps = np.empty((ns, n))
+    nl = (C.c_size_t * 2)(*(list(nel) + [0] * (2 - ns)))
+    e = C.c_double(0.0)
+    rc = lib.rscf_guess_step(ri.h, _colmajor(S).ctypes.data, _colmajor(H).ctypes.data, float(enuc), ns, nl,
+                             d0f.ctypes.data, int(prec), C.byref(e), d1.ctypes.data, C_.ctypes.data,
+                             eps.ctypes.data)
+    if rc != 0:
+        raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
+    d1 = np.stack([x.T.copy() for x in d1])
+    C_ = np.stack([x.T.copy() for x in C_])
+    return e.value, (d1[0] if ns == 1 else d1), (C_[0] if ns == 1 else C_), (eps[0] if ns == 1 else eps)
+
+
+def mcweeny(ri: "jk.DeviceRI", S, D, iterations: int = 3):
+    """McWeeny-purified closed-shell density (rscf_mcweeny); returns (D, electron count 2 tr(PS))."""
+    lib = _bind(jk.load_library())
+    out = np.empty_like(np.asarray(D, dtype=np.float64))
+    ne = C.c_double(0.0)
+    rc = lib.rscf_mcweeny(ri.h, _colmajor(S).ctypes.data, _colmajor(D).ctypes.data, int(iterations),
+                          out.ctypes.data, C.byref(ne))
+    if rc != 0:
+        raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
+    return out.T.copy(), ne.value
 
 
 def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctrl] = None,
-                 threshold: float = 1e-8, keep_b: bool = False) -> tuple:
-    """Integrals, B build and SCF for a rest_amd.ri.Molecule; returns (ScfOutcome, timings dict).
-    With keep_b the host copy of the device B is returned as timings['rimatr']."""
+                 threshold: float = 1e-8, keep_b: bool = False, guess: str = "sad") -> tuple:
+    """Integrals, B build, initial guess and SCF for a rest_amd.ri.Molecule; returns
+    (ScfOutcome, timings dict).  guess: "sad" (REST's default, src/ctrl_io/mod.rs:229) or "hcore";
+    an explicit d_guess density overrides it.  With keep_b the host copy of the device B is
+    returned as timings['rimatr']."""
+    from . import guess as gs
     t0 = time.perf_counter()
     S, T, V = mol.int1e()
     H = T + V
@@ -105,10 +172,20 @@ def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctr
     store = jk.STORE_FP64 | (jk.STORE_FP32 if prec == jk.PREC_FP32 else 0)
     binfo = mol.build_rimatr(ri_, threshold, store)
     t2 = time.perf_counter()
+    ginfo = {}
+    if d_guess is None and guess == "sad":
+        g = gs.sad_guess(mol, ri_, S, H)
+        d_guess = g.d1
+        ginfo = {"guess": "sad", "e_guess_d0": g.energy_d0}
+    elif d_guess is None and guess != "hcore":
+        raise ValueError(f"unknown guess {guess!r}")
+    else:
+        ginfo = {"guess": "density" if d_guess is not None else "hcore"}
+    t3 = time.perf_counter()
     ctrl = ctrl or default_ctrl()
     ctrl.jk_prec = prec
     out = rhf(ri_, S, H, mol.nuclear_repulsion(), mol.nelectron, d_guess, ctrl)
-    t3 = time.perf_counter()
+    t4 = time.perf_counter()
     extra = {"rimatr": ri_.get_rimatr(), "S": S, "H": H} if keep_b else {}
     ri_.close()
-    return out, {"t_1e": t1 - t0, "t_B": t2 - t1, "t_scf": t3 - t2, **binfo, **extra}
+    return out, {"t_1e": t1 - t0, "t_B": t2 - t1, "t_guess": t3 - t2, "t_scf": t4 - t3, **ginfo, **binfo, **extra}

@@ -1,5 +1,5 @@
 """CPU checks of the C-ABI library: it loads, exports every entry point that
-include/rest_jk.h declares, and its host-side argument checks work without a
+include/*.h declares, and its host-side argument checks work without a
 GPU (no compute call is made here)."""
 import ctypes as C
 import re
@@ -16,7 +16,7 @@ def header_symbols():
     syms = set()
     for h in sorted((ROOT / "include").glob("*.h")):
         text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
-        syms |= set(re.findall(r"\b((?:rjk|rsi|rscf)_[a-z0-9_]+)\s*\(", text))
+        syms |= set(re.findall(r"\b((?:rjk|rsi|rscf|rlg)_[a-z0-9_]+)\s*\(", text))
     return sorted(syms)
 
 

@@ -1,0 +1,113 @@
+"""GPU tests of the UHF driver (rscf_uhf), the density-guess step
+(rscf_guess_step) and the SAD initial guess (rest_amd/guess.py).
+
+Pins: the one-electron closed form (H atom: lowest eigenvalue of (H, S));
+UHF = RHF for a closed shell; device UHF vs the oracle UHF on the same B
+(≤ 1e-9 Eh); REST's pentacene run with its SAD guess
+(tests/golden/pentacene_rest_trace.json, from examples/PEN/RIHF/rest.out):
+the guess energy E(D0), the energy of the first density E(D1), and the
+converged energy.
+"""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from rest_amd import guess as gs
+from rest_amd import jk, ri
+from rest_amd import scf as rs
+from rest_amd import workload as wl
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def tight():
+    return rs.default_ctrl(scf_acc_etot=1e-11, scf_acc_rho=1e-9, scf_acc_eev=1e-8, max_scf_cycle=200)
+
+
+def device_setup(m):
+    S, T, V = m.int1e()
+    r = jk.DeviceRI(m.nbf, m.naux)
+    m.build_rimatr(r)
+    return r, S, T + V
+
+
+def test_uhf_one_electron_closed_form():
+    say("UHF H atom def2-SVP: E = lowest eigenvalue of (H, S)")
+    m = ri.Molecule(["H"], np.zeros((1, 3)))
+    r, S, H = device_setup(m)
+    with r:
+        out = rs.uhf(r, S, H, 0.0, 1, 0, ctrl=tight())
+    e_ref = sla.eigh(H, S, eigvals_only=True)[0]
+    say(f"  E {out.energy:.14f} closed form {e_ref:.14f}")
+    assert out.converged and abs(out.energy - e_ref) < 1e-11
+    np.testing.assert_allclose(out.dm[1], 0.0, atol=0)
+
+
+def test_uhf_closed_shell_equals_rhf():
+    say("(H2O)2 UHF(10,10) from the RHF density vs RHF")
+    m = ri.Molecule.water_cluster(wl.water_cluster(3, 2))
+    r, S, H = device_setup(m)
+    with r:
+        o_r = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, ctrl=tight())
+        o_u = rs.uhf(r, S, H, m.nuclear_repulsion(), 10, 10, d_guess=(0.5 * o_r.dm, 0.5 * o_r.dm), ctrl=tight())
+    say(f"  RHF {o_r.energy:.12f} UHF {o_u.energy:.12f}")
+    assert o_r.converged and o_u.converged and abs(o_u.energy - o_r.energy) < 1e-9
+
+
+@pytest.mark.parametrize("el,na,nb", [("O", 5, 3), ("C", 4, 2)])
+def test_uhf_atom_vs_oracle_identical_b(el, na, nb):
+    from oracle import scf_oracle as so
+    from tests.test_integrals import numpy_jk
+    say(f"UHF {el} ({na},{nb}) def2-SVP/JKFIT device vs oracle on the device B")
+    m = ri.Molecule([el], np.zeros((1, 3)))
+    r, S, H = device_setup(m)
+    with r:
+        out = rs.uhf(r, S, H, 0.0, na, nb, ctrl=tight())
+        B = r.get_rimatr()
+    ctrl = so.ScfCtrl(scf_acc_etot=1e-11, scf_acc_rho=1e-9, scf_acc_eev=1e-8, max_scf_cycle=200)
+    ref = so.uhf(S, H, B, na, nb, 0.0, ctrl, jk=numpy_jk(B, m.nbf))
+    say(f"  device {out.energy:.12f} ({out.iterations} it)  oracle {ref.energy:.12f} ({ref.iterations} it)")
+    assert out.converged and ref.converged and abs(out.energy - ref.energy) <= 1e-9
+
+
+def test_sad_pentacene_reproduces_rest_guess_and_energy():
+    trace = json.loads((GOLD / "pentacene_rest_trace.json").read_text())
+    lines = (GOLD / "pentacene.xyz").read_text().splitlines()[2:]
+    sym = [l.split()[0] for l in lines]
+    xyz = np.array([[float(x) for x in l.split()[1:]] for l in lines])
+    m = ri.Molecule.from_angstrom(sym, xyz, basis="STO-3G", auxbasis="def2-SV(P)-JKFIT")
+    say("pentacene SAD guess + SCF with REST defaults")
+    out, t = rs.run_molecule(m, guess="sad")
+    for el in ("C", "H"):
+        a = gs.atomic_density(el, "STO-3G", "def2-SV(P)-JKFIT")
+        say(f"  atom {el}: E {a.energy:.10f} ({a.iterations} it, converged {a.converged})")
+    e_log = [row["energy"] for row in out.log]
+    say(f"  E(D0) {t['e_guess_d0']:.8f} REST {trace['initial_guess_hf_energy']:.8f}")
+    say(f"  iterations {out.iterations} (REST {trace['converged_after']}); energies "
+        + " ".join(f"{e:.10f}" for e in e_log))
+    say(f"  REST energies " + " ".join(f"{e:.10f}" for e in trace['iteration_energies']))
+    assert out.converged
+    assert abs(out.energy - trace["final_energy"]) < 1e-8
+    assert abs(t["e_guess_d0"] - trace["initial_guess_hf_energy"]) < 1e-8
+
+
+def test_sad_water_cluster_beats_core_guess():
+    say("(H2O)4 SAD vs core-Hamiltonian guess")
+    m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
+    o_sad, t = rs.run_molecule(m, guess="sad", ctrl=tight())
+    o_core, _ = rs.run_molecule(m, guess="hcore", ctrl=tight())
+    say(f"  SAD {o_sad.energy:.12f} ({o_sad.iterations} it, E(D0) {t['e_guess_d0']:.8f})  "
+        f"hcore {o_core.energy:.12f} ({o_core.iterations} it)")
+    assert o_sad.converged and o_core.converged
+    assert abs(o_sad.energy - o_core.energy) < 1e-9
+    assert o_sad.iterations < o_core.iterations
