@@ -14,7 +14,8 @@
  * jk_prec = RJK_PREC_FP32 the J/K builds run on the FP32 copy of B until the
  * DIIS error max|FDS − SDF| falls below switch_diis_err, then switch to FP64
  * for the remaining iterations (the FP64 builds act as iterative
- * refinement); convergence is only declared on two consecutive FP64 energies.
+ * refinement) and restart DIIS from the first FP64 Fock matrix; convergence
+ * is only declared on two consecutive FP64 energies.
  * Diagonalisation, DIIS, the density and the energy are FP64 throughout.
  */
 #ifndef REST_SCF_H
@@ -88,6 +89,13 @@ int rscf_uhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t
  * RJK_PREC_FP32 to rank guesses cheaply).  mo_coeff / mo_energy may be NULL. */
 int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin, const size_t* nelec,
                     const double* d0, int prec, double* energy, double* d1, double* mo_coeff, double* mo_energy);
+
+/* The Hartree-Fock energy of a given density, E = E_nuc + ½ Σ_s Σ D_s (H + F_s)
+ * with F_s from the density-only J/K (evaluate_hf_total_energy after
+ * generate_hf_hamiltonian_for_guess, src/scf_io/mod.rs:1657-1713), without
+ * diagonalising.  nspin / nelec / dm as in rscf_guess_step. */
+int rscf_energy(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin, const size_t* nelec,
+                const double* dm, int prec, double* energy);
 
 /* McWeeny purification of a closed-shell density in the S metric (for a
  * predicted guess density; no REST counterpart): P = D/2,

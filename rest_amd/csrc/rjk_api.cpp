@@ -53,7 +53,7 @@ int ws_get(rjk_ctx* ctx, size_t bytes, void** out) {
     return RJK_OK;
 }
 
-PhaseScope::PhaseScope(rjk_ctx* c, int phase) : ctx(c), on(c->prof) {
+PhaseScope::PhaseScope(rjk_ctx* c, int phase, int prec, double work) : ctx(c), on(c->prof) {
     if (!on) return;
     if (!ctx->ev_free.empty()) {
         ev = ctx->ev_free.back();
@@ -63,6 +63,8 @@ PhaseScope::PhaseScope(rjk_ctx* c, int phase) : ctx(c), on(c->prof) {
         (void)hipEventCreate(&ev.b);
     }
     ev.phase = phase;
+    ev.prec = prec == RJK_PREC_FP64 ? 0 : 1;
+    ev.work = work;
     (void)hipEventRecord(ev.a, ctx->stream);
 }
 PhaseScope::~PhaseScope() {
@@ -297,8 +299,10 @@ extern "C" int rjk_vj_upper_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_
         PhaseScope ps(ctx, 4);
         RJK_HIP(ctx, launch_dm_weights(st, dm_dev, ctx->nbf, ctx->npair, ctx->ldb, ns, full_dm, w));
     }
+    const double sb = prec == RJK_PREC_FP64 ? 8.0 : 4.0;
+    const double jpass = (double)ctx->npair * ctx->naux * sb + 8.0 * ns * ctx->npair;   // one pass over B
     {
-        PhaseScope ps(ctx, 0);
+        PhaseScope ps(ctx, 0, prec, jpass);
         RJK_HIP(ctx, launch_jgamma(st, prec, b, ctx->ldb, ctx->naux, ns, w, nchunk, gpart));
     }
     {
@@ -306,7 +310,7 @@ extern "C" int rjk_vj_upper_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_
         RJK_HIP(ctx, launch_gamma_reduce(st, gpart, nchunk, ns, ctx->naux, gamma));
     }
     {
-        PhaseScope ps(ctx, 1);
+        PhaseScope ps(ctx, 1, prec, jpass);
         RJK_HIP(ctx, launch_jacc(st, prec, b, ctx->ldb, ctx->naux, ns, gamma, nsplit, jpart));
     }
     {
@@ -428,12 +432,13 @@ int k_run(rjk_ctx* ctx, const KPlan& kp, int ns, int prec, const void* b, const 
     }
     for (size_t P0 = 0; P0 < ctx->naux; P0 += kp.chunk) {
         size_t Pc = std::min(kp.chunk, ctx->naux - P0);
+        const double nb = (double)ctx->nbf;
         {
-            PhaseScope ps(ctx, 2);
+            PhaseScope ps(ctx, 2, prec, 2.0 * nb * nb * (double)kp.nk * (double)Pc);
             RJK_HIP(ctx, launch_khalf(st, prec, b, ctx->ldb, ctx->nbf, P0, Pc, kp, cocc, m));
         }
         {
-            PhaseScope ps(ctx, 3);
+            PhaseScope ps(ctx, 3, prec, nb * (nb + 1.0) * (double)kp.nk * (double)Pc);
             RJK_HIP(ctx, launch_ksyrk(st, prec, m, Pc, kp, ns, sign, P0 > 0 ? 1 : 0, kpart));
         }
     }
@@ -628,24 +633,54 @@ extern "C" int rjk_profile_enable(rjk_ctx* ctx, int on) {
     return RJK_OK;
 }
 
-extern "C" int rjk_profile_read(rjk_ctx* ctx, double* ms, uint64_t* launches) {
-    if (!ctx) return RJK_ERR_ARG;
+namespace {
+int profile_collect(rjk_ctx* ctx) {
     RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (auto& e : ctx->ev_pending) {
         float t = 0;
         if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) {
-            ctx->prof_ms[e.phase] += t;
-            ctx->prof_n[e.phase] += 1;
+            ctx->prof_ms[e.prec][e.phase] += t;
+            ctx->prof_n[e.prec][e.phase] += 1;
+            ctx->prof_work[e.prec][e.phase] += e.work;
         }
         ctx->ev_free.push_back(e);
     }
     ctx->ev_pending.clear();
+    return RJK_OK;
+}
+void profile_reset(rjk_ctx* ctx) {
+    for (int p = 0; p < 2; ++p)
+        for (int i = 0; i < kNumPhases; ++i) {
+            ctx->prof_ms[p][i] = 0;
+            ctx->prof_n[p][i] = 0;
+            ctx->prof_work[p][i] = 0;
+        }
+}
+}  // namespace
+
+extern "C" int rjk_profile_read(rjk_ctx* ctx, double* ms, uint64_t* launches) {
+    if (!ctx) return RJK_ERR_ARG;
+    int rc = profile_collect(ctx);
+    if (rc) return rc;
     for (int i = 0; i < kNumPhases; ++i) {
-        if (ms) ms[i] = ctx->prof_ms[i];
-        if (launches) launches[i] = ctx->prof_n[i];
-        ctx->prof_ms[i] = 0;
-        ctx->prof_n[i] = 0;
+        if (ms) ms[i] = ctx->prof_ms[0][i] + ctx->prof_ms[1][i];
+        if (launches) launches[i] = ctx->prof_n[0][i] + ctx->prof_n[1][i];
     }
+    profile_reset(ctx);
+    return RJK_OK;
+}
+
+extern "C" int rjk_profile_read_detail(rjk_ctx* ctx, double* ms, uint64_t* launches, double* work) {
+    if (!ctx) return RJK_ERR_ARG;
+    int rc = profile_collect(ctx);
+    if (rc) return rc;
+    for (int p = 0; p < 2; ++p)
+        for (int i = 0; i < kNumPhases; ++i) {
+            if (ms) ms[p * kNumPhases + i] = ctx->prof_ms[p][i];
+            if (launches) launches[p * kNumPhases + i] = ctx->prof_n[p][i];
+            if (work) work[p * kNumPhases + i] = ctx->prof_work[p][i];
+        }
+    profile_reset(ctx);
     return RJK_OK;
 }
 

@@ -42,9 +42,10 @@ struct rjk_ctx {
     std::string err;
     // phase timing
     bool prof = false;
-    double prof_ms[rjk::kNumPhases] = {0};
-    uint64_t prof_n[rjk::kNumPhases] = {0};
-    struct EvPair { hipEvent_t a, b; int phase; };
+    double prof_ms[2][rjk::kNumPhases] = {};     // [FP64, FP32][phase]
+    uint64_t prof_n[2][rjk::kNumPhases] = {};
+    double prof_work[2][rjk::kNumPhases] = {};   // algorithmic bytes (J phases) or FLOP (K phases)
+    struct EvPair { hipEvent_t a, b; int phase; int prec; double work; };
     std::vector<EvPair> ev_pending;
     std::vector<EvPair> ev_free;
 };
@@ -72,7 +73,7 @@ struct PhaseScope {
     rjk_ctx* ctx;
     rjk_ctx::EvPair ev;
     bool on;
-    PhaseScope(rjk_ctx* c, int phase);
+    PhaseScope(rjk_ctx* c, int phase, int prec = 0, double work = 0.0);
     ~PhaseScope();
 };
 

@@ -230,7 +230,8 @@ namespace {
 // E(D0), F C = S C ε and the new density, written to dm_out.
 int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h, double enuc, int nspin,
              const size_t nel[2], const double* d_guess, const rscf_ctrl* ctrl_in, bool guess_only,
-             rscf_result* result, double* mo_coeff, double* mo_energy, double* dm_out, double* iter_log) {
+             rscf_result* result, double* mo_coeff, double* mo_energy, double* dm_out, double* iter_log,
+             bool energy_only = false) {
     if (!S_h || !H_h) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": S and H are required");
     rscf_ctrl ctrl;
     rscf_default_ctrl(&ctrl);
@@ -419,6 +420,16 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     int prec = ctrl.jk_prec;
     double E = 0.0;
     SCF_RC(fock(Dp, guess_from_orbitals, prec, &E));
+    if (energy_only) {
+        SCF_HIP(hipStreamSynchronize(st));
+        if (result) {
+            *result = rscf_result{};
+            result->energy = E;
+            result->wall_seconds = now() - t_start;
+            result->jk_seconds = t_jk;
+        }
+        return RJK_OK;
+    }
     int prec_E = prec;
     for (int s = 0; s < nspin; ++s) {
         SCF_RC(diagonalise(F[s], s));
@@ -524,8 +535,14 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                 head = 0;
             }
             t_diis += now() - t0;
-            // precision switch: FP64 J/K from the next build on
-            if (prec != RJK_PREC_FP64 && diis_max < ctrl.switch_diis_err) prec = RJK_PREC_FP64;
+            // precision switch: FP64 J/K from the next build on.  The stored Fock
+            // matrices and error vectors carry the FP32 J/K error (about the size of
+            // the switch threshold), so DIIS restarts with FP64 vectors only.
+            if (prec != RJK_PREC_FP64 && diis_max < ctrl.switch_diis_err) {
+                prec = RJK_PREC_FP64;
+                nvec = 0;
+                head = 0;
+            }
         }
         prec_E = prec_used;
         if (prec_used == RJK_PREC_FP64) ++n_fp64;
@@ -622,6 +639,22 @@ extern "C" int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, d
     const int rc = scf_core(ctx, "rscf_guess_step", S, H, enuc, (int)nspin, nel, d0, &c, true, &r, mo_coeff,
                             mo_energy, d1, nullptr);
     if (!rc && energy) *energy = r.energy;
+    return rc;
+}
+
+extern "C" int rscf_energy(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin,
+                           const size_t* nelec, const double* dm, int prec, double* energy) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!nelec || !dm || !energy || (nspin != 1 && nspin != 2))
+        return rjk::fail(ctx, RJK_ERR_ARG, "rscf_energy: nspin must be 1 or 2, nelec, dm and energy non-NULL");
+    const size_t nel[2] = {nelec[0], nspin == 2 ? nelec[1] : 0};
+    rscf_ctrl c;
+    rscf_default_ctrl(&c);
+    c.jk_prec = prec;
+    rscf_result r{};
+    const int rc = scf_core(ctx, "rscf_energy", S, H, enuc, (int)nspin, nel, dm, &c, true, &r, nullptr, nullptr,
+                            nullptr, nullptr, true);
+    if (!rc) *energy = r.energy;
     return rc;
 }
 

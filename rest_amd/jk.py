@@ -44,12 +44,12 @@ EXPORTS = [
     "rjk_set_rimatr_dev", "rjk_set_ri3fn", "rjk_fill_rimatr_synthetic", "rjk_rimatr_dev", "rjk_get_rimatr",
     "rjk_vj_upper_with_rimatr_v_sync", "rjk_vj_upper_with_ri_v_sync", "rjk_vj_upper_dev",
     "rjk_vk_upper_with_ri_v_sync", "rjk_vk_upper_occ_dev", "rjk_vk_upper_with_rimatr_use_dm_only_sync",
-    "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_profile_enable", "rjk_profile_read", "rjk_copy_dev",
+    "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_profile_enable", "rjk_profile_read", "rjk_profile_read_detail", "rjk_copy_dev",
     # include/rest_ri.h
     "rsi_basis_create", "rsi_basis_destroy", "rsi_basis_nbf", "rsi_last_error", "rsi_int1e", "rsi_int2c2e",
     "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr", "rsi_cart2sph",
     # include/rest_scf.h
-    "rscf_default_ctrl", "rscf_rhf", "rscf_uhf", "rscf_guess_step", "rscf_mcweeny",
+    "rscf_default_ctrl", "rscf_rhf", "rscf_uhf", "rscf_guess_step", "rscf_energy", "rscf_mcweeny",
     # include/rest_lg.h
     "rlg_create", "rlg_destroy", "rlg_last_error", "rlg_num_params", "rlg_set_stream", "rlg_set_params",
     "rlg_get_params", "rlg_forward", "rlg_loss_grad", "rlg_adam_step", "rlg_reset_optimizer", "rlg_gemm_bf16",
@@ -108,6 +108,7 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rjk_vk_upper_with_ri_v_use_dm_only_sync": (i, [vp, vp, sz, d, i, vp]),
         "rjk_profile_enable": (i, [vp, i]),
         "rjk_profile_read": (i, [vp, pd, C.POINTER(C.c_uint64)]),
+        "rjk_profile_read_detail": (i, [vp, pd, C.POINTER(C.c_uint64), pd]),
         "rjk_copy_dev": (i, [vp, vp, vp, sz]),
         "rsi_basis_create": (i, [sz, vp, vp, vp, vp, vp, sz, vp, C.POINTER(vp)]),
         "rsi_basis_destroy": (i, [vp]),
@@ -289,6 +290,15 @@ class DeviceRI:
         n = (C.c_uint64 * 5)()
         self._check(self.lib.rjk_profile_read(self.h, ms, n))
         return {name: (ms[k], int(n[k])) for k, name in enumerate(self.PHASES)}
+
+    def profile_read_detail(self) -> dict:
+        """{(prec, phase): (ms, launches, algorithmic work)} with prec "fp64" / "fp32"."""
+        ms = (C.c_double * 10)()
+        n = (C.c_uint64 * 10)()
+        w = (C.c_double * 10)()
+        self._check(self.lib.rjk_profile_read_detail(self.h, ms, n, w))
+        return {(p, name): (ms[5 * pi + k], int(n[5 * pi + k]), w[5 * pi + k])
+                for pi, p in enumerate(("fp64", "fp32")) for k, name in enumerate(self.PHASES)}
 
     def copy_dev(self, dst_ptr: int, src_ptr: int, nbytes: int):
         self._check(self.lib.rjk_copy_dev(self.h, C.c_void_p(dst_ptr), C.c_void_p(src_ptr), int(nbytes)))

@@ -11,20 +11,28 @@ Representation
       A = B: e_z along v_A (all neighbours), e_x from the nearest neighbour.
   - Inputs (N_IN numbers): r_AB/10 and 1/r_AB (0 for A = B), the ordered
     element pair one-hot over ELEMENTS² and a same-atom flag, 16-bin Gaussian
-    radial neighbour features of A and of B per neighbour element, and v_A,
-    v_B in the pair frame with their lengths.
-  - Outputs: the pair's block of D − D_SAD (the converged closed-shell density
-    minus the superposition of atomic densities) in the pair frame, on a
-    MAXB × MAXB grid (rows: A's functions, columns: B's) with a mask for the
-    element pair's real block size, scaled by TARGET_SCALE.
+    radial neighbour features of A and of B per neighbour element, v_A and v_B
+    in the pair frame with their lengths, and the NNEIGH nearest neighbours of
+    A and of B (other than A and B) in the pair frame: coordinates / 5 bohr,
+    1/r and element one-hot.
+  - Outputs: the pair's block of D − D_sph in the pair frame, on a MAXB × MAXB
+    grid (rows: A's functions, columns: B's) with a mask for the element
+    pair's real block size, scaled by TARGET_SCALE.  D is the converged
+    closed-shell density; D_sph is the superposition of spherically averaged
+    atomic densities (SAD's atomic blocks averaged over rotations: within an
+    atom, blocks between two shells of equal l become tr/(2l+1) times the
+    identity, all others zero).  The average makes the target independent of
+    the lab-frame axis along which an open-shell atom's SCF put its partly
+    filled p shell.
   - Blocks are carried between the molecule frame and the pair frame by the
     real-solid-harmonic rotation matrices of the engine's own convention
     (rsi_cart2sph): D_loc = U_A D U_Bᵀ.
 
-Guess: D = D_SAD + predicted blocks (pairs beyond R_CUT keep D_SAD's zero),
-symmetrised, McWeeny-purified (rscf_mcweeny), and accepted only if its guess
-energy E(D) (rscf_guess_step) is below that of the SAD density; otherwise the
-SAD guess is used.
+Guess: D = D_sph + predicted blocks (pairs beyond R_CUT keep zero),
+symmetrised, scaled to the molecule's electron count (tr D S = N),
+McWeeny-purified (rscf_mcweeny), and accepted only if its energy E(D) is
+below the energy of SAD's first orbital density (REST's guess step on the SAD
+density, then rscf_energy); otherwise the SAD guess is used.
 
 The network runs on the device through include/rest_lg.h (BF16 matrix-core
 GEMMs, FP32 master weights, Adam); torch only provides device buffers.
@@ -55,7 +63,9 @@ R_ENV = 12.0                            # neighbour cut-off of the radial featur
 SIGMA_REF = 2.0                         # width of the environment-vector weights (bohr)
 MAXB = 14                               # largest per-atom block (O, C, N in def2-SVP)
 TARGET_SCALE = 4.0
-N_IN = 2 + len(ELEMENTS) ** 2 + 1 + 2 * len(ELEMENTS) * NBINS + 8
+NNEIGH = 4                              # nearest neighbours of A and of B described in the pair frame
+NB_FEAT = 4 + len(ELEMENTS)             # local x, y, z (/5 bohr), 1/r, element one-hot
+N_IN = 2 + len(ELEMENTS) ** 2 + 1 + 2 * len(ELEMENTS) * NBINS + 8 + 2 * NNEIGH * NB_FEAT
 N_OUT = MAXB * MAXB
 MODEL_DIR = Path(__file__).resolve().parent / "models"
 
@@ -112,27 +122,45 @@ def _eval_rows(l, P):
     return mono @ harmonic_rows(l).T
 
 
-def sph_rotation(l: int, R: np.ndarray) -> np.ndarray:
-    """D^l(R): for the molecule rotated by R (r → R r), shell-l coefficients transform as c' = D^l c."""
+_PINV: dict = {}
+
+
+def sph_rotations(l: int, Rs: np.ndarray) -> np.ndarray:
+    """D^l(R) for a stack of rotations Rs [k, 3, 3]: for the molecule rotated by R (r → R r),
+    shell-l coefficients transform as c' = D^l c.  Least squares of Y_m(Rᵀ p) = Σ_m' Y_m'(p) D_m'm
+    over fixed sample points p on the unit sphere (exact: the rows span an invariant space)."""
+    Rs = np.asarray(Rs, dtype=np.float64).reshape(-1, 3, 3)
     if l == 0:
-        return np.ones((1, 1))
+        return np.ones((len(Rs), 1, 1))
     P = _sample_points()
-    Phi = _eval_rows(l, P)
-    PhiR = _eval_rows(l, P @ R)          # Y_m(Rᵀ p)
-    return np.linalg.lstsq(Phi, PhiR, rcond=None)[0]
+    if l not in _PINV:
+        _PINV[l] = np.linalg.pinv(_eval_rows(l, P))
+    PR = np.einsum("pi,kij->kpj", P, Rs)              # rows (Rᵀ p)ᵀ = pᵀ R
+    ex = _cart_exps(l)
+    mono = np.stack([PR[..., 0] ** a * PR[..., 1] ** b * PR[..., 2] ** c for a, b, c in ex], axis=-1)
+    return _PINV[l] @ (mono @ harmonic_rows(l).T)
+
+
+def sph_rotation(l: int, R: np.ndarray) -> np.ndarray:
+    return sph_rotations(l, np.asarray(R)[None])[0]
+
+
+def atom_rotations(shell_ls, Rs: np.ndarray) -> np.ndarray:
+    """Block-diagonal rotations [k, n, n] of one atom's functions (its shells in basis order)."""
+    Rs = np.asarray(Rs, dtype=np.float64).reshape(-1, 3, 3)
+    Dl = {l: sph_rotations(l, Rs) for l in set(shell_ls)}
+    n = sum(2 * l + 1 for l in shell_ls)
+    U = np.zeros((len(Rs), n, n))
+    o = 0
+    for l in shell_ls:
+        k = 2 * l + 1
+        U[:, o:o + k, o:o + k] = Dl[l]
+        o += k
+    return U
 
 
 def atom_rotation(shell_ls, R: np.ndarray) -> np.ndarray:
-    """Block-diagonal rotation of one atom's functions (its shells in basis order)."""
-    blocks = [sph_rotation(l, R) for l in shell_ls]
-    n = sum(b.shape[0] for b in blocks)
-    U = np.zeros((n, n))
-    o = 0
-    for b in blocks:
-        k = b.shape[0]
-        U[o:o + k, o:o + k] = b
-        o += k
-    return U
+    return atom_rotations(shell_ls, np.asarray(R)[None])[0]
 
 
 def molecule_rotation(mol: "ri.Molecule", R: np.ndarray) -> np.ndarray:
@@ -143,6 +171,23 @@ def molecule_rotation(mol: "ri.Molecule", R: np.ndarray) -> np.ndarray:
         Ua = atom_rotation(ls, R)
         U[o:o + Ua.shape[0], o:o + Ua.shape[0]] = Ua
     return U
+
+
+def spherical_average(mol: "ri.Molecule", D: np.ndarray) -> np.ndarray:
+    """Rotation average of the atom-diagonal blocks of D; off-atom blocks are dropped."""
+    b = mol.orb
+    out = np.zeros_like(D)
+    for a in range(len(mol.symbols)):
+        idx = np.nonzero(b.shell_atom == a)[0]
+        for i in idx:
+            for j in idx:
+                li, lj = int(b.shell_l[i]), int(b.shell_l[j])
+                if li != lj:
+                    continue
+                oi, oj, k = int(b.shell_off[i]), int(b.shell_off[j]), 2 * li + 1
+                blk = D[oi:oi + k, oj:oj + k]
+                out[oi:oi + k, oj:oj + k] = np.trace(blk) / k * np.eye(k)
+    return out
 
 
 def atom_layout(mol: "ri.Molecule") -> list:
@@ -218,75 +263,170 @@ def radial_features(symbols, xyz: np.ndarray) -> np.ndarray:
     return out.reshape(n, -1)
 
 
+def _first_valid(cands: np.ndarray, z: np.ndarray) -> np.ndarray:
+    """Per row, unit(perp(first candidate whose component ⊥ z is not negligible)); cands [k, m, 3]."""
+    perp = cands - np.einsum("kmi,ki->km", cands, z)[:, :, None] * z[:, None, :]
+    pn = np.linalg.norm(perp, axis=-1)
+    ok = pn > 1e-3 * np.maximum(1.0, np.linalg.norm(cands, axis=-1))
+    if not ok.any(axis=1).all():
+        raise RuntimeError("no local frame")
+    first = np.argmax(ok, axis=1)
+    rows = np.arange(len(cands))
+    return perp[rows, first] / pn[rows, first][:, None]
+
+
+def frames(xyz: np.ndarray, pairs: np.ndarray) -> np.ndarray:
+    """Local frames [k, 3, 3] (rows e_x, e_y, e_z) of all pairs; the same frames as pair_frame."""
+    n = len(xyz)
+    d = xyz[None, :, :] - xyz[:, None, :]                  # d[a, c] = x_c − x_a
+    r = np.linalg.norm(d, axis=-1)
+    w = np.exp(-r ** 2 / SIGMA_REF ** 2)
+    np.fill_diagonal(w, 0.0)
+    v_all = np.einsum("ac,aci->ai", w, d)
+    A, B = pairs[:, 0], pairs[:, 1]
+    F = np.empty((len(pairs), 3, 3))
+    ex = np.broadcast_to(np.array([1.0, 0.0, 0.0]), (len(pairs), 3))
+    ey = np.broadcast_to(np.array([0.0, 1.0, 0.0]), (len(pairs), 3))
+    off = A != B
+    if off.any():
+        a, b = A[off], B[off]
+        z = d[a, b] / r[a, b][:, None]
+        va = v_all[a] - w[a, b][:, None] * d[a, b]
+        vb = v_all[b] - w[b, a][:, None] * d[b, a]
+        x = _first_valid(np.stack([va, vb, ex[off], ey[off]], axis=1), z)
+        F[off] = np.stack([x, np.cross(z, x), z], axis=1)
+    dg = ~off
+    if dg.any():
+        a = A[dg]
+        rr = r.copy()
+        np.fill_diagonal(rr, np.inf)
+        order = np.argsort(rr, axis=1, kind="stable")[:, :3]
+        vn = np.linalg.norm(v_all[a], axis=1)
+        near = d[a, order[a, 0]] if n > 1 else np.broadcast_to(np.array([0.0, 0.0, 1.0]), (len(a), 3))
+        zc = np.where((vn > 1e-8)[:, None], v_all[a], near)
+        z = zc / np.linalg.norm(zc, axis=1, keepdims=True)
+        nb = [d[a, order[a, j]] if j < n - 1 else np.zeros((len(a), 3)) for j in range(3)]
+        x = _first_valid(np.stack(nb + [ex[dg], ey[dg]], axis=1), z)
+        F[dg] = np.stack([x, np.cross(z, x), z], axis=1)
+    return F
+
+
 def descriptors(symbols, xyz: np.ndarray, pairs: np.ndarray):
     """(features [npair, N_IN] float32, frames [npair, 3, 3])."""
-    rad = radial_features(symbols, xyz)
-    el = [ELEMENTS.index(s) for s in symbols]
+    n = len(symbols)
     ne = len(ELEMENTS)
-    X = np.zeros((len(pairs), N_IN), dtype=np.float64)
-    F = np.empty((len(pairs), 3, 3))
-    for k, (a, b) in enumerate(pairs):
-        R = pair_frame(xyz, a, b)
-        F[k] = R
-        r = np.linalg.norm(xyz[b] - xyz[a])
-        o = 0
-        X[k, 0] = r / 10.0
-        X[k, 1] = 1.0 / r if a != b else 0.0
-        o = 2
-        X[k, o + el[a] * ne + el[b]] = 1.0
-        o += ne * ne
-        X[k, o] = 1.0 if a == b else 0.0
-        o += 1
-        X[k, o:o + rad.shape[1]] = rad[a]
-        o += rad.shape[1]
-        X[k, o:o + rad.shape[1]] = rad[b]
-        o += rad.shape[1]
-        va = R @ _env_vector(xyz, a, (a, b))
-        vb = R @ _env_vector(xyz, b, (a, b))
-        X[k, o:o + 3] = va
-        X[k, o + 3] = np.linalg.norm(va)
-        X[k, o + 4:o + 7] = vb
-        X[k, o + 7] = np.linalg.norm(vb)
+    el = np.array([ELEMENTS.index(s) for s in symbols])
+    rad = radial_features(symbols, xyz)
+    F = frames(xyz, pairs)
+    A, B = pairs[:, 0], pairs[:, 1]
+    k = len(pairs)
+    d = xyz[None, :, :] - xyz[:, None, :]
+    r = np.linalg.norm(d, axis=-1)
+    w = np.exp(-r ** 2 / SIGMA_REF ** 2)
+    np.fill_diagonal(w, 0.0)
+    v_all = np.einsum("ac,aci->ai", w, d)
+    same = A == B
+    rab = r[A, B]
+    X = np.zeros((k, N_IN))
+    X[:, 0] = rab / 10.0
+    X[:, 1] = np.where(same, 0.0, 1.0 / np.where(same, 1.0, rab))
+    o = 2
+    X[np.arange(k), o + el[A] * ne + el[B]] = 1.0
+    o += ne * ne
+    X[:, o] = same
+    o += 1
+    X[:, o:o + rad.shape[1]] = rad[A]
+    o += rad.shape[1]
+    X[:, o:o + rad.shape[1]] = rad[B]
+    o += rad.shape[1]
+    va = v_all[A] - np.where(same, 0.0, w[A, B])[:, None] * d[A, B]
+    vb = v_all[B] - np.where(same, 0.0, w[B, A])[:, None] * d[B, A]
+    la = np.einsum("kij,kj->ki", F, va)
+    lb = np.einsum("kij,kj->ki", F, vb)
+    X[:, o:o + 3] = la
+    X[:, o + 3] = np.linalg.norm(la, axis=1)
+    X[:, o + 4:o + 7] = lb
+    X[:, o + 7] = np.linalg.norm(lb, axis=1)
+    o += 8
+    # nearest neighbours of A and of B other than A and B, in the pair frame
+    rr = r.copy()
+    np.fill_diagonal(rr, np.inf)
+    order = np.argsort(rr, axis=1, kind="stable")[:, :NNEIGH + 1]      # [n, <= NNEIGH+1]
+    if order.shape[1] < NNEIGH + 1:   # fewer atoms: pad with the atom itself (never valid)
+        order = np.concatenate([order, np.repeat(np.arange(n)[:, None], NNEIGH + 1 - order.shape[1], axis=1)], axis=1)
+    for atom, other in ((A, B), (B, A)):
+        cand = order[atom]                                              # [k, NNEIGH+1]
+        valid = (cand != other[:, None]) & np.isfinite(rr[atom[:, None], cand])
+        sel = np.argsort(~valid, axis=1, kind="stable")[:, :NNEIGH]
+        c = np.take_along_axis(cand, sel, axis=1)
+        ok = np.take_along_axis(valid, sel, axis=1)
+        loc = np.einsum("kij,kmj->kmi", F, d[atom[:, None], c])       # [k, NNEIGH, 3]
+        feat = np.zeros((k, NNEIGH, NB_FEAT))
+        feat[..., 0:3] = loc / 5.0
+        feat[..., 3] = 1.0 / np.maximum(r[atom[:, None], c], 1e-6)
+        feat[np.arange(k)[:, None], np.arange(NNEIGH)[None, :], 4 + el[c]] = 1.0
+        feat[~ok] = 0.0
+        X[:, o:o + NNEIGH * NB_FEAT] = feat.reshape(k, -1)
+        o += NNEIGH * NB_FEAT
+    assert o == N_IN
     return X.astype(np.float32), F
 
 
 # ---------------------------------------------------------------------------
 # encoding densities as per-pair targets and back
 # ---------------------------------------------------------------------------
+def _groups(mol: "ri.Molecule", pairs: np.ndarray):
+    """Pairs grouped by ordered element pair: (rows, offsets A, offsets B, shells A, shells B)."""
+    lay = atom_layout(mol)
+    off = np.array([o for o, _ in lay])
+    sym = np.array(mol.symbols)
+    out = []
+    keys = sorted(set(zip(sym[pairs[:, 0]], sym[pairs[:, 1]])))
+    for ea, eb in keys:
+        rows = np.nonzero((sym[pairs[:, 0]] == ea) & (sym[pairs[:, 1]] == eb))[0]
+        la = lay[pairs[rows[0], 0]][1]
+        lb = lay[pairs[rows[0], 1]][1]
+        out.append((rows, off[pairs[rows, 0]], off[pairs[rows, 1]], la, lb))
+    return out
+
+
 def encode(mol: "ri.Molecule", D: np.ndarray, D_sad: np.ndarray):
-    """Training arrays (X, T, M) of one molecule with converged density D."""
+    """Training arrays (X, T, M) of one molecule with converged density D and baseline D_sad
+    (the spherically averaged SAD density in training)."""
     pairs = pair_list(mol.xyz_bohr)
     X, F = descriptors(mol.symbols, mol.xyz_bohr, pairs)
-    lay = atom_layout(mol)
     T = np.zeros((len(pairs), MAXB, MAXB), dtype=np.float32)
     M = np.zeros_like(T)
     dD = D - D_sad
-    for k, (a, b) in enumerate(pairs):
-        (oa, la), (ob, lb) = lay[a], lay[b]
-        Ua, Ub = atom_rotation(la, F[k]), atom_rotation(lb, F[k])
-        na, nb = Ua.shape[0], Ub.shape[0]
+    for rows, oa, ob, la, lb in _groups(mol, pairs):
+        Ua, Ub = atom_rotations(la, F[rows]), atom_rotations(lb, F[rows])
+        na, nb = Ua.shape[1], Ub.shape[1]
         if na > MAXB or nb > MAXB:
             raise ValueError(f"atom blocks above {MAXB} functions are not supported")
-        T[k, :na, :nb] = TARGET_SCALE * (Ua @ dD[oa:oa + na, ob:ob + nb] @ Ub.T)
-        M[k, :na, :nb] = 1.0
+        ia = oa[:, None] + np.arange(na)[None, :]
+        ib = ob[:, None] + np.arange(nb)[None, :]
+        blk = dD[ia[:, :, None], ib[:, None, :]]
+        T[rows, :na, :nb] = TARGET_SCALE * (Ua @ blk @ np.transpose(Ub, (0, 2, 1)))
+        M[rows, :na, :nb] = 1.0
     return X, T.reshape(len(pairs), -1), M.reshape(len(pairs), -1)
 
 
-def decode(mol: "ri.Molecule", pairs: np.ndarray, frames: np.ndarray, Y: np.ndarray, D_sad: np.ndarray):
-    """D = D_SAD + Σ predicted pair blocks rotated back, symmetric."""
-    lay = atom_layout(mol)
+def decode(mol: "ri.Molecule", pairs: np.ndarray, frames_: np.ndarray, Y: np.ndarray, D_sad: np.ndarray):
+    """D = D_sad + Σ predicted pair blocks rotated back, symmetric."""
     dD = np.zeros_like(D_sad)
-    Y = Y.reshape(len(pairs), MAXB, MAXB) / TARGET_SCALE
-    for k, (a, b) in enumerate(pairs):
-        (oa, la), (ob, lb) = lay[a], lay[b]
-        Ua, Ub = atom_rotation(la, frames[k]), atom_rotation(lb, frames[k])
-        na, nb = Ua.shape[0], Ub.shape[0]
-        blk = Ua.T @ Y[k, :na, :nb].astype(np.float64) @ Ub
-        if a == b:
-            dD[oa:oa + na, oa:oa + na] += 0.5 * (blk + blk.T)
-        else:
-            dD[oa:oa + na, ob:ob + nb] += blk
-            dD[ob:ob + nb, oa:oa + na] += blk.T
+    Y = np.asarray(Y, dtype=np.float64).reshape(len(pairs), MAXB, MAXB) / TARGET_SCALE
+    for rows, oa, ob, la, lb in _groups(mol, pairs):
+        Ua, Ub = atom_rotations(la, frames_[rows]), atom_rotations(lb, frames_[rows])
+        na, nb = Ua.shape[1], Ub.shape[1]
+        blk = np.transpose(Ua, (0, 2, 1)) @ Y[rows, :na, :nb] @ Ub
+        same = pairs[rows, 0] == pairs[rows, 1]
+        if same.any():   # only element-diagonal groups hold atom-diagonal blocks (square)
+            blk[same] = 0.5 * (blk[same] + np.transpose(blk[same], (0, 2, 1)))
+        ia = oa[:, None] + np.arange(na)[None, :]
+        ib = ob[:, None] + np.arange(nb)[None, :]
+        np.add.at(dD, (ia[:, :, None], ib[:, None, :]), blk)
+        nd = ~same
+        np.add.at(dD, (ib[nd][:, :, None], ia[nd][:, None, :]), np.transpose(blk[nd], (0, 2, 1)))
     return D_sad + dD
 
 
@@ -452,30 +592,42 @@ def predict_density(net: Net, mol: "ri.Molecule", D_sad: np.ndarray) -> np.ndarr
 class LearnedGuess:
     d1: np.ndarray              # density the SCF starts from
     kind: str                   # "learned" or "sad"
-    e_learned: float
-    e_sad: float
+    e_learned: float            # E(D) of the purified prediction
+    e_sad: float                # E(D1) of SAD's first orbital density
     electrons: float            # 2 tr(PS) of the purified prediction
     seconds: dict = field(default_factory=dict)
 
 
 def learned_guess(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, prec: int = jk.PREC_FP64,
-                  mcweeny_iterations: int = 3) -> LearnedGuess:
-    """Predict, purify, and keep the prediction only if its guess energy is below SAD's."""
+                  mcweeny_iterations: int = 6) -> LearnedGuess:
+    """Predict, purify, and keep the prediction only if its energy is below that of SAD's first
+    orbital density.
+
+    Both energies are of idempotent densities with the molecule's electron count, so both are
+    upper bounds of the converged energy and compare fairly (SAD's own D0 is not idempotent and
+    its energy can lie below the converged one).  The purified prediction then takes REST's
+    guess step (F(D), diagonalise, new density) like SAD's D0 does."""
     t0 = time.perf_counter()
     da, db = gs.sad_density(mol)
     d_sad = da + db
     t1 = time.perf_counter()
-    d_pred = predict_density(net, mol, d_sad)
+    d_pred = predict_density(net, mol, spherical_average(mol, d_sad))
+    d_pred *= mol.nelectron / float(np.sum(d_pred * S))
     t2 = time.perf_counter()
     d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
     t3 = time.perf_counter()
     enuc = mol.nuclear_repulsion()
     nel = (mol.nelectron,)
-    e_l, d1_l, _, _ = rs.guess_step(r, S, H, enuc, nel, d_pur, prec)
-    e_s, d1_s, _, _ = rs.guess_step(r, S, H, enuc, nel, d_sad, prec)
+    _, d1_s, _, _ = rs.guess_step(r, S, H, enuc, nel, d_sad, prec)
+    e_s = rs.energy(r, S, H, enuc, nel, d1_s, prec)
     t4 = time.perf_counter()
-    ok = np.isfinite(e_l) and abs(ne - mol.nelectron) < 0.5 and e_l < e_s
-    secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "guess_steps": t4 - t3}
+    ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
+    e_l, d1_l = float("inf"), None
+    if ok:
+        e_l, d1_l, _, _ = rs.guess_step(r, S, H, enuc, nel, d_pur, prec)
+        ok = bool(np.isfinite(e_l)) and e_l < e_s
+    t5 = time.perf_counter()
+    secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "learned_step": t5 - t4}
     return LearnedGuess(d1_l if ok else d1_s, "learned" if ok else "sad", e_l, e_s, ne, secs)
 
 
@@ -502,7 +654,7 @@ def make_training_data(indices, base_seed: int = 4096, basis="def2-SVP", auxbasi
             log(f"cluster {i}: SCF did not converge, skipped")
             continue
         da, db = gs.sad_density(mol)
-        X, T, M = encode(mol, out.dm, da + db)
+        X, T, M = encode(mol, out.dm, spherical_average(mol, da + db))
         Xs.append(X)
         Ts.append(T)
         Ms.append(M)

@@ -45,6 +45,8 @@ def _bind(lib):
     lib.rscf_guess_step.restype = C.c_int
     lib.rscf_guess_step.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, C.c_int, C.POINTER(C.c_double), vp,
                                     vp, vp]
+    lib.rscf_energy.restype = C.c_int
+    lib.rscf_energy.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, C.c_int, C.POINTER(C.c_double)]
     lib.rscf_mcweeny.restype = C.c_int
     lib.rscf_mcweeny.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_double)]
     lib._rscf_bound = True
@@ -135,9 +137,9 @@ def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PRE
     eps = np.empty((ns, n))
     nl = (C.c_size_t * 2)(*(list(nel) + [0] * (2 - ns)))
     e = C.c_double(0.0)
-    rc = lib.rscf_guess_step(ri.h, _colmajor(S).ctypes.data, _colmajor(H).ctypes.data, float(enuc), ns, nl,
-                             d0f.ctypes.data, int(prec), C.byref(e), d1.ctypes.data, C_.ctypes.data,
-                             eps.ctypes.data)
+    Sf, Hf = _colmajor(S), _colmajor(H)      # kept alive across the call
+    rc = lib.rscf_guess_step(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, d0f.ctypes.data, int(prec),
+                             C.byref(e), d1.ctypes.data, C_.ctypes.data, eps.ctypes.data)
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
     d1 = np.stack([x.T.copy() for x in d1])
@@ -145,13 +147,29 @@ def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PRE
     return e.value, (d1[0] if ns == 1 else d1), (C_[0] if ns == 1 else C_), (eps[0] if ns == 1 else eps)
 
 
+def energy(ri: "jk.DeviceRI", S, H, enuc: float, nel, dm, prec: int = jk.PREC_FP64) -> float:
+    """HF energy of a density without diagonalising (rscf_energy); nel = (nelec,) or (nα, nβ)."""
+    lib = _bind(jk.load_library())
+    ns = len(nel)
+    dms = [dm] if ns == 1 else list(dm)
+    df = np.ascontiguousarray(np.stack([_colmajor(d) for d in dms]))
+    nl = (C.c_size_t * 2)(*(list(nel) + [0] * (2 - ns)))
+    Sf, Hf = _colmajor(S), _colmajor(H)
+    e = C.c_double(0.0)
+    rc = lib.rscf_energy(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, df.ctypes.data, int(prec),
+                         C.byref(e))
+    if rc != 0:
+        raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
+    return e.value
+
+
 def mcweeny(ri: "jk.DeviceRI", S, D, iterations: int = 3):
     """McWeeny-purified closed-shell density (rscf_mcweeny); returns (D, electron count 2 tr(PS))."""
     lib = _bind(jk.load_library())
     out = np.empty_like(np.asarray(D, dtype=np.float64))
     ne = C.c_double(0.0)
-    rc = lib.rscf_mcweeny(ri.h, _colmajor(S).ctypes.data, _colmajor(D).ctypes.data, int(iterations),
-                          out.ctypes.data, C.byref(ne))
+    Sf, Df = _colmajor(S), _colmajor(D)      # kept alive across the call
+    rc = lib.rscf_mcweeny(ri.h, Sf.ctypes.data, Df.ctypes.data, int(iterations), out.ctypes.data, C.byref(ne))
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
     return out.T.copy(), ne.value

@@ -106,7 +106,7 @@ def test_training_fits_a_smooth_target():
     T = (np.tanh(X @ W1) @ W2).astype(np.float32)
     M = np.ones_like(T)
     net = lg.Net(lg.N_IN, lg.N_OUT, 256, 3, 4096, seed=1)
-    hist = lg.train(net, X, T, M, epochs=12, batch=2048, lr=2e-3, log=say)
+    hist = lg.train(net, X, T, M, epochs=40, batch=512, lr=2e-3, log=say)
     var = float(np.mean((T - T.mean(axis=0)) ** 2))
     say(f"  target variance {var:.3e}, final val {hist['val'][-1]:.3e}")
     assert hist["val"][-1] < var / 20

@@ -51,6 +51,18 @@ def test_frames_are_right_handed_and_descriptors_invariant():
     np.testing.assert_allclose(F2, F1 @ R.T, atol=1e-10)   # frames rotate with the molecule
 
 
+def test_vectorised_frames_equal_the_per_pair_definition():
+    for xyz in (wl.water_cluster(5, 3, oh_jitter=0.05) / ri.ANG, wl.water_cluster(2, 1) / ri.ANG,
+                np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]])):
+        pairs = lg.pair_list(xyz)
+        F = lg.frames(xyz, pairs)
+        for k, (a, b) in enumerate(pairs):
+            np.testing.assert_allclose(F[k], lg.pair_frame(xyz, a, b), atol=1e-12)
+    X, _ = lg.descriptors(["H", "H"], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]]),
+                          np.array([[0, 0], [0, 1], [1, 1]]))
+    assert X.shape == (3, lg.N_IN) and np.all(np.isfinite(X))
+
+
 def test_targets_are_equivariant_and_round_trip():
     xyz = wl.water_cluster(6, 2, oh_jitter=0.05)
     m1 = ri.Molecule.water_cluster(xyz)
@@ -82,3 +94,21 @@ def test_training_clusters_are_seeded_and_perturbed():
     assert 2 <= n <= 24
     oh = np.linalg.norm(a[1::3] - a[0::3], axis=1)
     assert np.all(np.abs(oh - 0.9572) <= 0.0501)
+
+
+def test_spherical_average_is_rotation_invariant_and_keeps_electrons():
+    """The baseline of the targets: invariant under rotating the molecule (U D_sph Uᵀ is the rotated
+    molecule's D_sph even for an anisotropic atomic block), and it keeps tr(D S) per atom."""
+    xyz = wl.water_cluster(6, 1, oh_jitter=0.05)
+    m1 = ri.Molecule.water_cluster(xyz)
+    R = rotation(9)
+    m2 = ri.Molecule.water_cluster(xyz @ R.T)
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((m1.nbf, m1.nbf))
+    D = A @ A.T                                  # anisotropic blocks
+    U = lg.molecule_rotation(m1, R)
+    np.testing.assert_allclose(U @ lg.spherical_average(m1, D) @ U.T, lg.spherical_average(m2, U @ D @ U.T),
+                               atol=1e-10)
+    np.testing.assert_allclose(lg.spherical_average(m1, lg.spherical_average(m1, D)), lg.spherical_average(m1, D))
+    # same-shell traces are kept (the normalised functions of one shell share S_ii = 1)
+    np.testing.assert_allclose(np.trace(lg.spherical_average(m1, D)), np.trace(D), rtol=1e-12)
