@@ -1,16 +1,23 @@
-"""Benchmark of the MI355X RI-V J/K build (one J + one K = one SCF-iteration's
-worth of the hot path) at the (H2O)64 def2-SVP / def2-SVP-JKFIT shape.
+"""Benchmarks of the MI355X RI-V J/K path.  Prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload h2o64|h2o16]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode scf|jk] ...
 
-Prints ONE JSON line.  B is synthetic (seeded hash generator, values
-(2u−1)·sqrt(3/naux)) and resident in HBM before timing starts; D = C Cᵀ from
-the seeded orthonormal Gaussian C; K runs the occupied-orbital route with
-occupation 1.  All arithmetic FP64.
+--mode scf (default): the headline metric, SCF time to 1e-8 Eh for
+  (H2O)64 def2-SVP / def2-SVP-JKFIT (water cluster seed 2): plain FP64 from
+  REST's SAD guess vs mixed precision (FP32 J/K until the DIIS error is below
+  1e-5, then FP64) from the learned guess, on the same GPU and the same
+  resident B.  One step = one plain-FP64 SCF plus one mixed+learned SCF;
+  `value` is their time ratio (plain / mixed+learned).  Integrals, the B
+  build and the atomic SAD densities are outside the timed region and
+  reported separately.
+--mode jk: one J + one K build (one SCF iteration's worth of the hot path)
+  on synthetic B (seeded hash generator) at the (H2O)64 or (H2O)16 shape,
+  D = C Cᵀ from the seeded orthonormal Gaussian C, occupied-route K.
 
-`roofline` is for the kernel that dominates the step (measured live with HIP
-events on the launch stream); `cpu_baseline` times the oracle's
-reference-structured CPU J/K on a bounded sample of the same workload.
+`roofline` is for the kernel that dominates the timed region, from the
+library's live HIP-event timing on the launch stream and the algorithmic work
+of each launch; `cpu_baseline` times the oracle's reference-structured CPU
+J/K on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -37,7 +44,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="scf", choices=["scf", "jk"])
     ap.add_argument("--workload", default="h2o64", choices=["h2o64", "h2o16"])
+    ap.add_argument("--model", default=str(ROOT / "rest_amd" / "models" / "learned_guess_h2o.npz"))
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
                     help="fp64: plain FP64 path; fp32: FP32 copy of B, FP32 MFMA K, FP64 accumulation of J")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -98,7 +107,149 @@ def main():
               "the J/K path would shard over the auxiliary index P with one all-reduce of the packed "
               "J/K partials, which is not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "scf":
+        return main_scf(args)
+    return main_jk(args)
 
+
+def roofline_from_detail(detail: dict):
+    """Dominant kernel of the timed region and its achieved rate from the library's per-launch work."""
+    best = None
+    for (prec, phase), (ms, n, work) in detail.items():
+        if phase in PHASE_KERNEL and n and (best is None or ms > best[2]):
+            best = (prec, phase, ms, n, work)
+    prec, phase, ms, n, work = best
+    sec = ms / 1e3
+    if phase.startswith("j_"):
+        ach, peak, unit, bound = work / sec / 1e9, HBM_PEAK_GBS, "GB/s", "hbm"
+    else:
+        ach, peak, unit, bound = work / sec / 1e12, MFMA_PEAK_TFS[prec], "TFLOP/s", "mfma"
+    kernel = PHASE_KERNEL[phase] + ("<double>" if prec == "fp64" else "<float>")
+    return {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
+            "traffic": load_traffic(f"{PHASE_KERNEL[phase]}_{prec}_scf"), "kernel": kernel,
+            "launches": n, "avg_launch_ms": ms / n, "algorithmic_per_launch": work / n}
+
+
+def main_scf(args):
+    import torch
+    from rest_amd import guess as gs
+    from rest_amd import jk, ri
+    from rest_amd import learned as lg
+    from rest_amd import scf as rs
+    from rest_amd import workload as wl
+
+    n, seed = (64, 2) if args.workload == "h2o64" else (16, 1)
+    model = Path(args.model)
+    if not model.exists():
+        raise SystemExit(f"bench.py: learned-guess model {model} is missing (train it with "
+                         "`python -m rest_amd.train_guess --out {model}`)")
+    t0 = time.perf_counter()
+    mol = ri.Molecule.water_cluster(wl.water_cluster(seed, n))
+    S, T, V = mol.int1e()
+    H = T + V
+    enuc = mol.nuclear_repulsion()
+    t1 = time.perf_counter()
+    r = jk.DeviceRI(mol.nbf, mol.naux)
+    stream = torch.cuda.Stream(device=torch.device("cuda:0"))
+    torch.cuda.set_stream(stream)
+    r.set_stream(stream.cuda_stream)
+    binfo = mol.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32)
+    t2 = time.perf_counter()
+    net = lg.Net.load(model)
+    net.set_stream(stream.cuda_stream)
+    gs.sad_density(mol)            # atomic SCFs (cached per element)
+    t3 = time.perf_counter()
+
+    def plain():
+        da, db = gs.sad_density(mol)
+        e0, _, c1, _ = rs.guess_step(r, S, H, enuc, (mol.nelectron,), da + db, jk.PREC_FP64)
+        out = rs.rhf(r, S, H, enuc, mol.nelectron, None, rs.default_ctrl(jk_prec=jk.PREC_FP64), c_guess=c1)
+        return out, {"guess": "sad", "e_guess": e0}
+
+    def mixed():
+        g = lg.learned_guess(net, mol, r, S, H, jk.PREC_FP32)
+        out = rs.rhf(r, S, H, enuc, mol.nelectron, None, rs.default_ctrl(jk_prec=jk.PREC_FP32), c_guess=g.c1)
+        return out, {"guess": g.kind, "e_learned": g.e_learned, "e_sad": g.e_sad, "guess_seconds": g.seconds}
+
+    for _ in range(args.warmup):
+        plain()
+        mixed()
+    r.profile_enable(True)
+    r.profile_read_detail()   # reset
+    tp, tm, runs = [], [], []
+    torch.cuda.synchronize()
+    tw0 = time.perf_counter()
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        op, ip = plain()
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        om, im = mixed()
+        torch.cuda.synchronize()
+        c = time.perf_counter()
+        tp.append(b - a)
+        tm.append(c - b)
+        runs.append((op, ip, om, im))
+    wall = time.perf_counter() - tw0
+    detail_prof = r.profile_read_detail()
+    r.profile_enable(False)
+    op, ip, om, im = runs[-1]
+    t_plain, t_mixed = float(np.median(tp)), float(np.median(tm))
+    roofline = roofline_from_detail(detail_prof)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        shape = wl.H2O_64 if n == 64 else wl.H2O_16
+        c_occ = wl.random_orthonormal(seed, shape.nbf, shape.nocc)
+        cb = cpu_baseline(shape, seed, c_occ, c_occ @ c_occ.T, args.cpu_threads)
+        per_build = cb["j_seconds_scaled"] + cb["k_seconds_scaled"]
+        cpu = {"value": per_build * (op.iterations + 1), "unit": "s (plain FP64 SCF, J/K only)",
+               "cores": cb["cores"], "kind": cb["kind"],
+               "sample": cb["sample"] + f"; times the {op.iterations + 1} J/K builds of the plain FP64 SCF "
+                                        "(diagonalisation and DIIS not included)",
+               "gpu_plain_scf_s": t_plain}
+    out = {
+        "metric": "SCF time to 1e-8 Eh, (H2O)64 def2-SVP: mixed+learned vs plain FP64, same GPU"
+                  if n == 64 else "SCF time to 1e-8 Eh, (H2O)16 def2-SVP: mixed+learned vs plain FP64, same GPU",
+        "value": t_plain / t_mixed,
+        "unit": "x (plain FP64 SCF seconds / mixed+learned SCF seconds)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * (t_plain + t_mixed),
+        "wall_s_timed_region": wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (plain); f32->f64 J/K, f64 elsewhere (mixed)",
+        "data": "synthetic (seeded water cluster geometry; integrals from this build's engine)",
+        "config": {"workload": f"(H2O){n} seed {seed}, def2-SVP / def2-SVP-JKFIT, nbf={mol.nbf} naux={mol.naux} "
+                               f"nocc={mol.nelectron // 2}; RI-HF to |dE| <= 1e-8 Eh, ||dD||/nbf <= 1e-5, "
+                               "||d eps|| <= 1e-5 (REST defaults); FP64+FP32 B resident in HBM",
+                   "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nelectron // 2},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": {
+            "plain_fp64": {"scf_s": t_plain, "iterations": op.iterations, "energy": op.energy,
+                           "converged": op.converged, "jk_s": op.jk_seconds, "diag_s": op.diag_seconds, **ip},
+            "mixed_learned": {"scf_s": t_mixed, "iterations": om.iterations, "fp64_iterations": om.fp64_iterations,
+                              "energy": om.energy, "converged": om.converged, "jk_s": om.jk_seconds,
+                              "diag_s": om.diag_seconds, **im},
+            "dE_mixed_minus_plain": om.energy - op.energy,
+            "all_steps_plain_s": tp, "all_steps_mixed_s": tm,
+            "setup_s": {"int1e": t1 - t0, "b_build": t2 - t1, "b_build_parts": binfo, "sad_atoms_and_model": t3 - t2},
+            "kernels": {f"{p}:{ph}": {"ms": ms, "launches": k, "work": w} for (p, ph), (ms, k, w) in
+                        detail_prof.items() if k},
+            "model": getattr(net, "meta", {}),
+        },
+    }
+    print(json.dumps(out))
+    net.close()
+    r.close()
+
+
+def main_jk(args):
     import torch
     from rest_amd import jk
     from rest_amd import workload as wl

@@ -86,9 +86,21 @@ int rscf_uhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t
  * orbitals, which is where REST's scf() starts.  nspin 1: nelec[0] is the
  * electron count, d0 and d1 are nbf²; nspin 2: nelec = {nα, nβ}, d0 and d1
  * are 2×nbf².  prec: J/K arithmetic (RJK_PREC_FP64 as in REST, or
- * RJK_PREC_FP32 to rank guesses cheaply).  mo_coeff / mo_energy may be NULL. */
+ * RJK_PREC_FP32 to rank guesses cheaply).  mo_coeff / mo_energy may be NULL.
+ * energy_d1 (may be NULL) receives E(D1), the energy REST prints as "The
+ * total energy ... by the initial guess" at the start of scf() (:3195-3199),
+ * from the occupied-orbital J/K of the new orbitals. */
 int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin, const size_t* nelec,
-                    const double* d0, int prec, double* energy, double* d1, double* mo_coeff, double* mo_energy);
+                    const double* d0, int prec, double* energy, double* d1, double* mo_coeff, double* mo_energy,
+                    double* energy_d1);
+
+/* rscf_rhf started from guess orbitals instead of a density: c_guess is
+ * nbf×nbf column-major, its first nelec/2 columns occupied.  The first Fock
+ * matrix uses the occupied-orbital K, as REST's scf() does after its guess
+ * step (generate_hf_hamiltonian with eigenvectors present, :3193). */
+int rscf_rhf_orbitals(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nelec,
+                      const double* c_guess, const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff,
+                      double* mo_energy, double* dm, double* iter_log);
 
 /* The Hartree-Fock energy of a given density, E = E_nuc + ½ Σ_s Σ D_s (H + F_s)
  * with F_s from the density-only J/K (evaluate_hf_total_energy after

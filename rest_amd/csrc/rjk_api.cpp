@@ -402,9 +402,20 @@ int k_plan_ws(rjk_ctx* ctx, KPlan& kp, int ns, int prec, void** cocc, void** sig
     size_t nsplit = round_up(std::max<size_t>(1, target / (ntiles * ns)), 8);
     const size_t cap = std::max<size_t>(8, ((ctx->ws_limit / 4) / slot_bytes) / 8 * 8);
     nsplit = std::min(nsplit, cap);
-    const size_t fixed = small + nsplit * slot_bytes;
-    const size_t avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
+    size_t fixed = small + nsplit * slot_bytes;
+    size_t avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
     size_t chunk = perP ? std::min(ctx->naux, std::max<size_t>(1, avail / perP)) : ctx->naux;
+    if (prec == RJK_PREC_FP32 && kp.nk > 0) {
+        // FP32 accumulators: each split sums at most kFp32Chain products (pps auxiliary
+        // columns × nk) before its partial joins the FP64 sum, so rounding does not grow
+        // with naux.  Up to half the workspace goes to the partials.
+        const size_t pps = std::max<size_t>(1, kFp32Chain / kp.nk);
+        const size_t smax = std::max<size_t>(8, ((ctx->ws_limit / 2) / slot_bytes) / 8 * 8);
+        fixed = small + smax * slot_bytes;
+        avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
+        chunk = std::min({ctx->naux, std::max<size_t>(1, avail / perP), smax * pps});
+        nsplit = std::min(smax, round_up((chunk + pps - 1) / pps, 8));
+    }
     kp.nsplit = nsplit;
     kp.chunk = chunk;
     Carve cv;

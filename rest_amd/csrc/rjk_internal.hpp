@@ -13,6 +13,7 @@
 
 namespace rjk {
 
+constexpr size_t kFp32Chain = 4096;   // products per FP32 accumulator in the K SYRK before FP64
 constexpr size_t kLdAlign = 64;      // elements; device B columns start 512-B aligned (FP64)
 constexpr int kNumPhases = 5;        // J-gamma, J-acc, K-half, K-syrk, K-other
 constexpr int kTileMN = 128;         // nbf padding for the K tiles

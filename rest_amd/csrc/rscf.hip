@@ -2,7 +2,7 @@
 //
 // One J/K build per iteration through the context's resident B; Fock,
 // energy, DIIS and the density in HBM (FP64, rocBLAS GEMMs and small
-// kernels); F C = S C ε by rocSOLVER dsygvd.  Mirrors REST's scf()
+// kernels); F C = S C ε through X = S^{-1/2} and rocSOLVER dsyevd.  Mirrors REST's scf()
 // (src/scf_io/mod.rs:3190-3268) with prepare_next_input (:2947-3091),
 // generate_diis_error_vector (:3094-3136), diis_solver (:3138-3188) and
 // check_scf_convergence (:1875-1951).
@@ -115,6 +115,13 @@ __global__ void k_mix(const double* __restrict__ A, const double* __restrict__ B
                       double* __restrict__ out) {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n2) out[t] = A[t] + alpha * (B[t] - A[t]);
+}
+
+// out[:, j] = U[:, j] · w[j]   (column-major n × n)
+__global__ void k_scale_columns(const double* __restrict__ U, const double* __restrict__ w, size_t n,
+                                double* __restrict__ out) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n * n) out[t] = U[t] * w[t / n];
 }
 
 __global__ void k_add(const double* __restrict__ A, const double* __restrict__ B, size_t n2, double* __restrict__ out) {
@@ -231,7 +238,7 @@ namespace {
 int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h, double enuc, int nspin,
              const size_t nel[2], const double* d_guess, const rscf_ctrl* ctrl_in, bool guess_only,
              rscf_result* result, double* mo_coeff, double* mo_energy, double* dm_out, double* iter_log,
-             bool energy_only = false) {
+             bool energy_only = false, const double* c_guess = nullptr, double* energy_d1 = nullptr) {
     if (!S_h || !H_h) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": S and H are required");
     rscf_ctrl ctrl;
     rscf_default_ctrl(&ctrl);
@@ -262,6 +269,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     if (ctrl.jk_prec != RJK_PREC_FP64 && ctrl.jk_prec != RJK_PREC_FP32)
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64 or FP32");
     if (guess_only && !d_guess) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": a guess density is required");
+    if (d_guess && c_guess) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": give a guess density or orbitals");
     const double t_start = now();
     SCF_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;
@@ -274,7 +282,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     DevMem mem;
     double* S = mem.get(ctx, n2, &rc);
     double* H = mem.get(ctx, n2, &rc);
-    double* Bw = mem.get(ctx, n2, &rc);     // S copy consumed by dsygvd
+    double* Bw = mem.get(ctx, n2, &rc);     // eigensolver operand
     double* T1 = mem.get(ctx, n2, &rc);
     double* T2 = mem.get(ctx, n2, &rc);
     double* Dt = mem.get(ctx, n2, &rc);     // total density for J (UHF)
@@ -336,17 +344,51 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         eps[s].assign(n, 0.0);
         eps_prev[s].assign(n, 0.0);
     }
+    // Orthogonaliser X = S^{-1/2} = U s^{-1/2} Uᵀ, once: every F C = S C ε then becomes
+    // F' = X F X (two GEMMs), F' C' = C' ε (rocSOLVER dsyevd), C = X C' (one GEMM).
+    // Same eigenpairs as REST's generalised solver (dspgvx, src/scf_io/mod.rs:1851),
+    // without re-factorising S in every iteration.
+    double* X = mem.get(ctx, n2, &rc);
+    if (rc) return rc;
+    {
+        SCF_HIP(hipMemcpyAsync(Bw, S, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        SCF_BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, Bw, (rocblas_int)n,
+                                  W[0], Ework, (rocblas_int*)info_d));
+        int info = 0;
+        std::vector<double> sv(n);
+        SCF_HIP(hipMemcpyAsync(&info, info_d, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCF_HIP(hipMemcpyAsync(sv.data(), W[0], n * sizeof(double), hipMemcpyDeviceToHost, st));
+        SCF_HIP(hipStreamSynchronize(st));
+        if (info != 0) return rjk::fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsyevd(S) info = " + std::to_string(info));
+        if (!(sv[0] > 0.0))
+            return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": the overlap matrix is not positive definite");
+        for (double& v : sv) v = 1.0 / std::sqrt(v);
+        SCF_HIP(hipMemcpyAsync(W[0], sv.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scale_columns, g2, b256, 0, st, Bw, W[0], n, T1);   // T1 = U s^{-1/2}
+        SCF_HIP(hipGetLastError());
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, T1, (rocblas_int)n, Bw, (rocblas_int)n, &zero, X,
+                               (rocblas_int)n));
+        SCF_HIP(hipStreamSynchronize(st));   // sv is a host temporary
+    }
     auto diagonalise = [&](const double* Fm, int s) -> int {
         const double t0 = now();
-        SCF_HIP(hipMemcpyAsync(A[s], Fm, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        SCF_HIP(hipMemcpyAsync(Bw, S, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
-        SCF_BLAS(rocsolver_dsygvd(h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n,
-                                  A[s], (rocblas_int)n, Bw, (rocblas_int)n, W[s], Ework, (rocblas_int*)info_d));
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, Fm, (rocblas_int)n, X, (rocblas_int)n, &zero, T1,
+                               (rocblas_int)n));
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, X, (rocblas_int)n, T1, (rocblas_int)n, &zero, Bw,
+                               (rocblas_int)n));
+        SCF_BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, Bw, (rocblas_int)n,
+                                  W[s], Ework, (rocblas_int*)info_d));
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, X, (rocblas_int)n, Bw, (rocblas_int)n, &zero, A[s],
+                               (rocblas_int)n));
         int info = 0;
         SCF_HIP(hipMemcpyAsync(&info, info_d, sizeof(int), hipMemcpyDeviceToHost, st));
         SCF_HIP(hipMemcpyAsync(eps[s].data(), W[s], n * sizeof(double), hipMemcpyDeviceToHost, st));
         SCF_HIP(hipStreamSynchronize(st));
-        if (info != 0) return rjk::fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsygvd info = " + std::to_string(info));
+        if (info != 0) return rjk::fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsyevd info = " + std::to_string(info));
         t_diag += now() - t0;
         return RJK_OK;
     };
@@ -410,6 +452,12 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     if (d_guess) {
         for (int s = 0; s < nspin; ++s)
             SCF_HIP(hipMemcpyAsync(Dp[s], d_guess + (size_t)s * n2, n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    } else if (c_guess) {   // guess orbitals: their density, first Fock by the occupied route
+        for (int s = 0; s < nspin; ++s) {
+            SCF_HIP(hipMemcpyAsync(A[s], c_guess + (size_t)s * n2, n2 * sizeof(double), hipMemcpyHostToDevice, st));
+            SCF_RC(density(s, Dp[s]));
+        }
+        guess_from_orbitals = true;
     } else {   // core-Hamiltonian guess
         for (int s = 0; s < nspin; ++s) {
             SCF_RC(diagonalise(H, s));
@@ -436,7 +484,10 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         SCF_RC(density(s, D[s]));
     }
     if (guess_only) {
+        double e1 = 0.0;
+        if (energy_d1) SCF_RC(fock(D, true, prec, &e1));   // energy of the new orbitals' density
         SCF_HIP(hipStreamSynchronize(st));
+        if (energy_d1) *energy_d1 = e1;
         if (result) {
             *result = rscf_result{};
             result->energy = E;
@@ -625,9 +676,19 @@ extern "C" int rscf_uhf(rjk_ctx* ctx, const double* S, const double* H, double e
                     iter_log);
 }
 
+extern "C" int rscf_rhf_orbitals(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nelec,
+                                 const double* c_guess, const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff,
+                                 double* mo_energy, double* dm, double* iter_log) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!c_guess) return rjk::fail(ctx, RJK_ERR_ARG, "rscf_rhf_orbitals: guess orbitals are required");
+    const size_t nel[2] = {nelec, 0};
+    return scf_core(ctx, "rscf_rhf_orbitals", S, H, enuc, 1, nel, nullptr, ctrl, false, result, mo_coeff, mo_energy,
+                    dm, iter_log, false, c_guess);
+}
+
 extern "C" int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin,
                                const size_t* nelec, const double* d0, int prec, double* energy, double* d1,
-                               double* mo_coeff, double* mo_energy) {
+                               double* mo_coeff, double* mo_energy, double* energy_d1) {
     if (!ctx) return RJK_ERR_ARG;
     if (!nelec || !d0 || (nspin != 1 && nspin != 2))
         return rjk::fail(ctx, RJK_ERR_ARG, "rscf_guess_step: nspin must be 1 or 2, nelec and d0 non-NULL");
@@ -637,7 +698,7 @@ extern "C" int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, d
     c.jk_prec = prec;
     rscf_result r{};
     const int rc = scf_core(ctx, "rscf_guess_step", S, H, enuc, (int)nspin, nel, d0, &c, true, &r, mo_coeff,
-                            mo_energy, d1, nullptr);
+                            mo_energy, d1, nullptr, false, nullptr, energy_d1);
     if (!rc && energy) *energy = r.energy;
     return rc;
 }

@@ -105,6 +105,7 @@ class Guess:
     d0: np.ndarray          # assembled guess density (RHF: D_α + D_β)
     energy_d0: float        # E(D0), REST's "Initial guess HF energy"
     d1: np.ndarray          # density of the orbitals of F(D0): where the SCF starts
+    c1: np.ndarray = None   # those orbitals (the SCF's first Fock uses them, as REST's scf() does)
     kind: str = "sad"
 
 
@@ -112,5 +113,5 @@ def sad_guess(mol: "ri.Molecule", r: "jk.DeviceRI", S: np.ndarray, H: np.ndarray
     """REST's SAD initial guess for a closed-shell molecule whose B is resident in r."""
     da, db = sad_density(mol)
     d0 = da + db
-    e0, d1, _, _ = rs.guess_step(r, S, H, mol.nuclear_repulsion(), (mol.nelectron,), d0)
-    return Guess(d0, e0, d1, "sad")
+    e0, d1, c1, _ = rs.guess_step(r, S, H, mol.nuclear_repulsion(), (mol.nelectron,), d0)
+    return Guess(d0, e0, d1, c1, "sad")

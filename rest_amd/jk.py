@@ -49,7 +49,7 @@ EXPORTS = [
     "rsi_basis_create", "rsi_basis_destroy", "rsi_basis_nbf", "rsi_last_error", "rsi_int1e", "rsi_int2c2e",
     "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr", "rsi_cart2sph",
     # include/rest_scf.h
-    "rscf_default_ctrl", "rscf_rhf", "rscf_uhf", "rscf_guess_step", "rscf_energy", "rscf_mcweeny",
+    "rscf_default_ctrl", "rscf_rhf", "rscf_rhf_orbitals", "rscf_uhf", "rscf_guess_step", "rscf_energy", "rscf_mcweeny",
     # include/rest_lg.h
     "rlg_create", "rlg_destroy", "rlg_last_error", "rlg_num_params", "rlg_set_stream", "rlg_set_params",
     "rlg_get_params", "rlg_forward", "rlg_loss_grad", "rlg_adam_step", "rlg_reset_optimizer", "rlg_gemm_bf16",

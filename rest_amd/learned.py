@@ -590,7 +590,7 @@ def predict_density(net: Net, mol: "ri.Molecule", D_sad: np.ndarray) -> np.ndarr
 
 @dataclass
 class LearnedGuess:
-    d1: np.ndarray              # density the SCF starts from
+    c1: np.ndarray              # orbitals the SCF starts from (rscf_rhf_orbitals)
     kind: str                   # "learned" or "sad"
     e_learned: float            # E(D) of the purified prediction
     e_sad: float                # E(D1) of SAD's first orbital density
@@ -618,17 +618,16 @@ def learned_guess(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, prec: in
     t3 = time.perf_counter()
     enuc = mol.nuclear_repulsion()
     nel = (mol.nelectron,)
-    _, d1_s, _, _ = rs.guess_step(r, S, H, enuc, nel, d_sad, prec)
-    e_s = rs.energy(r, S, H, enuc, nel, d1_s, prec)
+    _, _, c1_s, _, e_s = rs.guess_step(r, S, H, enuc, nel, d_sad, prec, want_e1=True)
     t4 = time.perf_counter()
     ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
-    e_l, d1_l = float("inf"), None
+    e_l, c1_l = float("inf"), None
     if ok:
-        e_l, d1_l, _, _ = rs.guess_step(r, S, H, enuc, nel, d_pur, prec)
+        e_l, _, c1_l, _ = rs.guess_step(r, S, H, enuc, nel, d_pur, prec)
         ok = bool(np.isfinite(e_l)) and e_l < e_s
     t5 = time.perf_counter()
     secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "learned_step": t5 - t4}
-    return LearnedGuess(d1_l if ok else d1_s, "learned" if ok else "sad", e_l, e_s, ne, secs)
+    return LearnedGuess(c1_l if ok else c1_s, "learned" if ok else "sad", e_l, e_s, ne, secs)
 
 
 # ---------------------------------------------------------------------------

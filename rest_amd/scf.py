@@ -39,12 +39,15 @@ def _bind(lib):
     lib.rscf_rhf.restype = C.c_int
     lib.rscf_rhf.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, C.POINTER(Ctrl), C.POINTER(Result), vp, vp,
                              vp, vp]
+    lib.rscf_rhf_orbitals.restype = C.c_int
+    lib.rscf_rhf_orbitals.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, C.POINTER(Ctrl), C.POINTER(Result),
+                                      vp, vp, vp, vp]
     lib.rscf_uhf.restype = C.c_int
     lib.rscf_uhf.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, C.c_size_t, vp, C.POINTER(Ctrl),
                              C.POINTER(Result), vp, vp, vp, vp]
     lib.rscf_guess_step.restype = C.c_int
     lib.rscf_guess_step.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, C.c_int, C.POINTER(C.c_double), vp,
-                                    vp, vp]
+                                    vp, vp, C.POINTER(C.c_double)]
     lib.rscf_energy.restype = C.c_int
     lib.rscf_energy.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, C.c_int, C.POINTER(C.c_double)]
     lib.rscf_mcweeny.restype = C.c_int
@@ -82,7 +85,7 @@ def _colmajor(M):
     return np.ascontiguousarray(np.asarray(M, dtype=np.float64).T)
 
 
-def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals) -> ScfOutcome:
+def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals, c_guess=None) -> ScfOutcome:
     lib = _bind(jk.load_library())
     n = ri.nbf
     ns = len(nel)
@@ -98,7 +101,11 @@ def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals) -> ScfOutcome:
     dm = np.empty((ns, n, n)) if want_orbitals else None
     log = np.zeros((ctrl.max_scf_cycle, len(LOG_FIELDS)))
     ptr = lambda a: None if a is None else a.ctypes.data
-    if ns == 1:
+    cg = None if c_guess is None else _colmajor(c_guess)
+    if ns == 1 and cg is not None:
+        rc = lib.rscf_rhf_orbitals(ri.h, ptr(Sf), ptr(Hf), float(enuc), int(nel[0]), ptr(cg), C.byref(ctrl),
+                                   C.byref(res), ptr(C_), ptr(eps), ptr(dm), ptr(log))
+    elif ns == 1:
         rc = lib.rscf_rhf(ri.h, ptr(Sf), ptr(Hf), float(enuc), int(nel[0]), ptr(dg), C.byref(ctrl), C.byref(res),
                           ptr(C_), ptr(eps), ptr(dm), ptr(log))
     else:
@@ -114,9 +121,11 @@ def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals) -> ScfOutcome:
 
 
 def rhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nelec: int,
-        d_guess: Optional[np.ndarray] = None, ctrl: Optional[Ctrl] = None, want_orbitals: bool = True) -> ScfOutcome:
-    """Closed-shell RI-HF on ri's resident B (rscf_rhf)."""
-    return _run(ri, S, H, enuc, (nelec,), d_guess, ctrl, want_orbitals)
+        d_guess: Optional[np.ndarray] = None, ctrl: Optional[Ctrl] = None, want_orbitals: bool = True,
+        c_guess: Optional[np.ndarray] = None) -> ScfOutcome:
+    """Closed-shell RI-HF on ri's resident B: from a guess density (rscf_rhf), guess orbitals
+    (c_guess, rscf_rhf_orbitals) or the core Hamiltonian."""
+    return _run(ri, S, H, enuc, (nelec,), d_guess, ctrl, want_orbitals, c_guess)
 
 
 def uhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nalpha: int, nbeta: int,
@@ -125,8 +134,9 @@ def uhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nalpha: in
     return _run(ri, S, H, enuc, (nalpha, nbeta), d_guess, ctrl, want_orbitals)
 
 
-def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PREC_FP64):
-    """REST's density-guess step (rscf_guess_step): returns (E(D0), D1, C, ε); nel = (nelec,) or (nα, nβ)."""
+def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PREC_FP64, want_e1: bool = False):
+    """REST's density-guess step (rscf_guess_step): returns (E(D0), D1, C, ε), plus E(D1) when
+    want_e1; nel = (nelec,) or (nα, nβ)."""
     lib = _bind(jk.load_library())
     n = ri.nbf
     ns = len(nel)
@@ -137,14 +147,17 @@ def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PRE
     eps = np.empty((ns, n))
     nl = (C.c_size_t * 2)(*(list(nel) + [0] * (2 - ns)))
     e = C.c_double(0.0)
+    e1 = C.c_double(0.0)
     Sf, Hf = _colmajor(S), _colmajor(H)      # kept alive across the call
     rc = lib.rscf_guess_step(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, d0f.ctypes.data, int(prec),
-                             C.byref(e), d1.ctypes.data, C_.ctypes.data, eps.ctypes.data)
+                             C.byref(e), d1.ctypes.data, C_.ctypes.data, eps.ctypes.data,
+                             C.byref(e1) if want_e1 else None)
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
     d1 = np.stack([x.T.copy() for x in d1])
     C_ = np.stack([x.T.copy() for x in C_])
-    return e.value, (d1[0] if ns == 1 else d1), (C_[0] if ns == 1 else C_), (eps[0] if ns == 1 else eps)
+    out = (e.value, (d1[0] if ns == 1 else d1), (C_[0] if ns == 1 else C_), (eps[0] if ns == 1 else eps))
+    return out + (e1.value,) if want_e1 else out
 
 
 def energy(ri: "jk.DeviceRI", S, H, enuc: float, nel, dm, prec: int = jk.PREC_FP64) -> float:
@@ -191,9 +204,10 @@ def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctr
     binfo = mol.build_rimatr(ri_, threshold, store)
     t2 = time.perf_counter()
     ginfo = {}
+    c_guess = None
     if d_guess is None and guess == "sad":
         g = gs.sad_guess(mol, ri_, S, H)
-        d_guess = g.d1
+        c_guess = g.c1
         ginfo = {"guess": "sad", "e_guess_d0": g.energy_d0}
     elif d_guess is None and guess != "hcore":
         raise ValueError(f"unknown guess {guess!r}")
@@ -202,7 +216,7 @@ def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctr
     t3 = time.perf_counter()
     ctrl = ctrl or default_ctrl()
     ctrl.jk_prec = prec
-    out = rhf(ri_, S, H, mol.nuclear_repulsion(), mol.nelectron, d_guess, ctrl)
+    out = rhf(ri_, S, H, mol.nuclear_repulsion(), mol.nelectron, d_guess, ctrl, c_guess=c_guess)
     t4 = time.perf_counter()
     extra = {"rimatr": ri_.get_rimatr(), "S": S, "H": H} if keep_b else {}
     ri_.close()

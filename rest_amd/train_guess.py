@@ -42,12 +42,12 @@ def evaluate(net, n: int, seeds, prec_mixed: bool = False) -> list:
             m.build_rimatr(r, 1e-8, store)
             t0 = time.perf_counter()
             da, db = gs.sad_density(m)
-            e0, d1, _, _ = rs.guess_step(r, S, H, m.nuclear_repulsion(), (m.nelectron,), da + db)
-            o_sad = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, d1, rs.default_ctrl())
+            e0, _, c1, _ = rs.guess_step(r, S, H, m.nuclear_repulsion(), (m.nelectron,), da + db)
+            o_sad = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, None, rs.default_ctrl(), c_guess=c1)
             t1 = time.perf_counter()
             g = lg.learned_guess(net, m, r, S, H, jk.PREC_FP32 if prec_mixed else jk.PREC_FP64)
             ctrl = rs.default_ctrl(jk_prec=jk.PREC_FP32 if prec_mixed else jk.PREC_FP64)
-            o_l = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, g.d1, ctrl)
+            o_l = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, None, ctrl, c_guess=g.c1)
             t2 = time.perf_counter()
         row = {"n": n, "seed": seed, "sad_iterations": o_sad.iterations, "sad_energy": o_sad.energy,
                "sad_guess_energy": e0, "sad_seconds": t1 - t0, "learned_kind": g.kind,

@@ -129,7 +129,7 @@ def test_learned_guess_falls_back_to_sad_for_a_useless_network():
         m.build_rimatr(r)
         g = lg.learned_guess(net, m, r, S, H)
         ctrl = rs.default_ctrl(scf_acc_etot=1e-11, scf_acc_rho=1e-9, scf_acc_eev=1e-8, max_scf_cycle=200)
-        out = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, g.d1, ctrl)
+        out = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, None, ctrl, c_guess=g.c1)
         ref, _ = rs.run_molecule(m, guess="sad", ctrl=rs.default_ctrl(scf_acc_etot=1e-11, scf_acc_rho=1e-9,
                                                                         scf_acc_eev=1e-8, max_scf_cycle=200))
     say(f"  kind {g.kind}: E_learned {g.e_learned:.6f} E_sad {g.e_sad:.6f} electrons {g.electrons:.4f}; "
