@@ -414,7 +414,7 @@ int k_plan_ws(rjk_ctx* ctx, KPlan& kp, int ns, int prec, void** cocc, void** sig
         fixed = small + smax * slot_bytes;
         avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
         chunk = std::min({ctx->naux, std::max<size_t>(1, avail / perP), smax * pps});
-        nsplit = std::min(smax, round_up((chunk + pps - 1) / pps, 8));
+        nsplit = std::min(smax, std::max(nsplit, round_up((chunk + pps - 1) / pps, 8)));
     }
     kp.nsplit = nsplit;
     kp.chunk = chunk;

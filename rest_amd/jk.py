@@ -50,6 +50,11 @@ EXPORTS = [
     "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr", "rsi_cart2sph",
     # include/rest_scf.h
     "rscf_default_ctrl", "rscf_rhf", "rscf_rhf_orbitals", "rscf_uhf", "rscf_guess_step", "rscf_energy", "rscf_mcweeny",
+    # include/rest_batch.h
+    "rjk_batch_create", "rjk_batch_destroy", "rjk_batch_last_error", "rjk_batch_size", "rjk_batch_bytes_b",
+    "rjk_batch_set_stream", "rjk_batch_set_rimatr", "rjk_batch_fill_synthetic", "rjk_batch_set_inputs",
+    "rjk_batch_get_outputs", "rjk_batch_run", "rjk_batch_synchronize", "rjk_batch_autotune", "rjk_batch_config",
+    "rjk_batch_set_config",
     # include/rest_lg.h
     "rlg_create", "rlg_destroy", "rlg_last_error", "rlg_num_params", "rlg_set_stream", "rlg_set_params",
     "rlg_get_params", "rlg_forward", "rlg_loss_grad", "rlg_adam_step", "rlg_reset_optimizer", "rlg_gemm_bf16",
@@ -120,6 +125,21 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rsi_int3c2e_packed_dev": (i, [vp, vp, vp, sz, sz, vp]),
         "rsi_cart2sph": (i, [i, vp]),
         "rsi_build_rimatr": (i, [vp, vp, vp, d, C.c_uint, C.POINTER(sz), pd]),
+        "rjk_batch_create": (i, [i, sz, vp, vp, vp, i, C.POINTER(vp)]),
+        "rjk_batch_destroy": (i, [vp]),
+        "rjk_batch_last_error": (C.c_char_p, [vp]),
+        "rjk_batch_size": (sz, [vp]),
+        "rjk_batch_bytes_b": (sz, [vp]),
+        "rjk_batch_set_stream": (i, [vp, vp]),
+        "rjk_batch_set_rimatr": (i, [vp, sz, vp]),
+        "rjk_batch_fill_synthetic": (i, [vp, C.c_uint64]),
+        "rjk_batch_set_inputs": (i, [vp, sz, vp, vp, vp]),
+        "rjk_batch_get_outputs": (i, [vp, sz, vp, vp]),
+        "rjk_batch_run": (i, [vp, d, d]),
+        "rjk_batch_synchronize": (i, [vp]),
+        "rjk_batch_autotune": (i, [vp, i, pd]),
+        "rjk_batch_config": (i, [vp, C.POINTER(i), C.POINTER(i)]),
+        "rjk_batch_set_config": (i, [vp, i, i]),
         "rlg_create": (i, [i, sz, sz, sz, sz, sz, C.POINTER(vp)]),
         "rlg_destroy": (i, [vp]),
         "rlg_last_error": (C.c_char_p, [vp]),
@@ -302,6 +322,92 @@ class DeviceRI:
 
     def copy_dev(self, dst_ptr: int, src_ptr: int, nbytes: int):
         self._check(self.lib.rjk_copy_dev(self.h, C.c_void_p(dst_ptr), C.c_void_p(src_ptr), int(nbytes)))
+
+
+class Batch:
+    """J/K of many problems in grouped persistent launches replayed from a hipGraph (include/rest_batch.h)."""
+
+    def __init__(self, nbf, naux, nocc, prec: int = PREC_FP64, device: int = 0):
+        self.lib = load_library()
+        self.nbf = [int(x) for x in nbf]
+        self.naux = [int(x) for x in naux]
+        self.nocc = [int(x) for x in nocc]
+        n = len(self.nbf)
+        arr = lambda v: (C.c_size_t * n)(*v)
+        h = C.c_void_p()
+        rc = self.lib.rjk_batch_create(int(device), n, arr(self.nbf), arr(self.naux), arr(self.nocc), int(prec),
+                                       C.byref(h))
+        if rc != RJK_OK:
+            raise RJKError(rc, "rjk_batch_create failed")
+        self.h = h
+        self.prec = prec
+
+    def _check(self, rc):
+        if rc != RJK_OK:
+            raise RJKError(rc, self.lib.rjk_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rjk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def bytes_b(self) -> int:
+        return int(self.lib.rjk_batch_bytes_b(self.h))
+
+    def set_stream(self, stream_ptr: int):
+        self._check(self.lib.rjk_batch_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def set_rimatr(self, i: int, rimatr: np.ndarray):
+        """rimatr indexed [p, P] (npair, naux), as DeviceRI.set_rimatr."""
+        buf = np.ascontiguousarray(np.asarray(rimatr, dtype=np.float64).T)
+        self._check(self.lib.rjk_batch_set_rimatr(self.h, i, _ptr(buf)))
+
+    def fill_synthetic(self, seed: int):
+        self._check(self.lib.rjk_batch_fill_synthetic(self.h, C.c_uint64(seed)))
+
+    def set_inputs(self, i: int, dm: np.ndarray, eigv: np.ndarray, occ: np.ndarray):
+        d = np.ascontiguousarray(np.asarray(dm, dtype=np.float64).T)
+        c = np.ascontiguousarray(np.asarray(eigv, dtype=np.float64)[:, :self.nocc[i]].T)
+        o = np.ascontiguousarray(np.asarray(occ, dtype=np.float64)[:self.nocc[i]])
+        self._check(self.lib.rjk_batch_set_inputs(self.h, i, _ptr(d), _ptr(c), _ptr(o)))
+
+    def get_outputs(self, i: int):
+        n = self.nbf[i] * (self.nbf[i] + 1) // 2
+        vj, vk = np.empty(n), np.empty(n)
+        self._check(self.lib.rjk_batch_get_outputs(self.h, i, _ptr(vj), _ptr(vk)))
+        return vj, vk
+
+    def run(self, scale_j: float = 1.0, scale_k: float = 1.0):
+        self._check(self.lib.rjk_batch_run(self.h, float(scale_j), float(scale_k)))
+
+    def synchronize(self):
+        self._check(self.lib.rjk_batch_synchronize(self.h))
+
+    def autotune(self, reps: int = 2) -> float:
+        ms = C.c_double(0.0)
+        self._check(self.lib.rjk_batch_autotune(self.h, int(reps), C.byref(ms)))
+        return ms.value
+
+    def config(self):
+        cut, w = C.c_int(0), C.c_int(0)
+        self._check(self.lib.rjk_batch_config(self.h, C.byref(cut), C.byref(w)))
+        return cut.value, w.value
+
+    def set_config(self, khalf_cut: int, waves_per_cu: int):
+        self._check(self.lib.rjk_batch_set_config(self.h, int(khalf_cut), int(waves_per_cu)))
 
 
 # ---------------------------------------------------------------------------
