@@ -189,12 +189,20 @@ def mcweeny(ri: "jk.DeviceRI", S, D, iterations: int = 3):
 
 
 def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctrl] = None,
-                 threshold: float = 1e-8, keep_b: bool = False, guess: str = "sad") -> tuple:
+                 threshold: float = 1e-8, keep_b: bool = False, guess: str = "sad", guess_path=None,
+                 chkfile=None) -> tuple:
     """Integrals, B build, initial guess and SCF for a rest_amd.ri.Molecule; returns
-    (ScfOutcome, timings dict).  guess: "sad" (REST's default, src/ctrl_io/mod.rs:229) or "hcore";
-    an explicit d_guess density overrides it.  With keep_b the host copy of the device B is
-    returned as timings['rimatr']."""
+    (ScfOutcome, timings dict).  guess (REST's initial_guess, src/initial_guess/mod.rs:15-72):
+      "sad"   — REST's default (src/ctrl_io/mod.rs:229);
+      "hcore" — core Hamiltonian;
+      "file"  — the density of a REST guess file (HDF5 dataset "init_guess") at guess_path,
+                then the guess step (initial_guess_from_hdf5guess, :75-87);
+      "chk"   — the orbitals of a REST chkfile (group "scf") at guess_path (restart, :89-128).
+    An explicit d_guess density overrides guess.  chkfile: write REST's chkfile (scf/e_tot,
+    scf/mo_coeff, scf/mo_energy; save_chkfile, src/post_scf_analysis/mod.rs:47-103) at the end.
+    With keep_b the host copy of the device B is returned as timings['rimatr']."""
     from . import guess as gs
+    from . import h5
     t0 = time.perf_counter()
     S, T, V = mol.int1e()
     H = T + V
@@ -203,21 +211,32 @@ def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctr
     store = jk.STORE_FP64 | (jk.STORE_FP32 if prec == jk.PREC_FP32 else 0)
     binfo = mol.build_rimatr(ri_, threshold, store)
     t2 = time.perf_counter()
-    ginfo = {}
     c_guess = None
-    if d_guess is None and guess == "sad":
+    if d_guess is not None:
+        ginfo = {"guess": "density"}
+    elif guess == "sad":
         g = gs.sad_guess(mol, ri_, S, H)
         c_guess = g.c1
         ginfo = {"guess": "sad", "e_guess_d0": g.energy_d0}
-    elif d_guess is None and guess != "hcore":
-        raise ValueError(f"unknown guess {guess!r}")
+    elif guess == "file":
+        d0 = h5.read_init_guess(guess_path, mol.nbf, 1)[0]
+        e0, _, c_guess, _ = guess_step(ri_, S, H, mol.nuclear_repulsion(), (mol.nelectron,), d0)
+        ginfo = {"guess": "file", "e_guess_d0": e0}
+    elif guess == "chk":
+        _, cs, _ = h5.read_chkfile(guess_path, mol.nbf, mol.nbf, 1)
+        c_guess = cs[0]
+        ginfo = {"guess": "chk"}
+    elif guess == "hcore":
+        ginfo = {"guess": "hcore"}
     else:
-        ginfo = {"guess": "density" if d_guess is not None else "hcore"}
+        raise ValueError(f"unknown guess {guess!r}")
     t3 = time.perf_counter()
     ctrl = ctrl or default_ctrl()
     ctrl.jk_prec = prec
     out = rhf(ri_, S, H, mol.nuclear_repulsion(), mol.nelectron, d_guess, ctrl, c_guess=c_guess)
     t4 = time.perf_counter()
+    if chkfile is not None:
+        h5.write_chkfile(chkfile, out.energy, [out.mo_coeff], [out.mo_energy])
     extra = {"rimatr": ri_.get_rimatr(), "S": S, "H": H} if keep_b else {}
     ri_.close()
     return out, {"t_1e": t1 - t0, "t_B": t2 - t1, "t_guess": t3 - t2, "t_scf": t4 - t3, **ginfo, **binfo, **extra}

@@ -123,3 +123,29 @@ def test_sad_water_cluster_beats_core_guess():
     assert o_sad.converged and o_core.converged
     assert abs(o_sad.energy - o_core.energy) < 1e-9
     assert o_sad.iterations < o_core.iterations
+
+
+def test_restart_from_pyscf_chkfile_and_own_chkfile(tmp_path):
+    """REST's restart path from a chkfile written by another program (PySCF's pentacene run,
+    tests/golden/pen_pyscf_restart.chk): converged in a few iterations to REST's energy; then the
+    chkfile this build writes restarts its own SCF."""
+    trace = json.loads((GOLD / "pentacene_rest_trace.json").read_text())
+    lines = (GOLD / "pentacene.xyz").read_text().splitlines()[2:]
+    sym = [l.split()[0] for l in lines]
+    xyz = np.array([[float(x) for x in l.split()[1:]] for l in lines])
+    m = ri.Molecule.from_angstrom(sym, xyz, basis="STO-3G", auxbasis="def2-SV(P)-JKFIT")
+    say("pentacene restarted from PySCF's chkfile")
+    out, _ = rs.run_molecule(m, guess="chk", guess_path=GOLD / "pen_pyscf_restart.chk",
+                             chkfile=tmp_path / "own.chk")
+    say(f"  E {out.energy:.10f} in {out.iterations} iterations (REST {trace['final_energy']})")
+    assert out.converged and out.iterations <= 5
+    assert abs(out.energy - trace["final_energy"]) < 1e-8
+    out2, _ = rs.run_molecule(m, guess="chk", guess_path=tmp_path / "own.chk")
+    say(f"  own chkfile restart: {out2.iterations} iterations")
+    assert out2.converged and out2.iterations <= 2 and abs(out2.energy - out.energy) < 1e-9
+    # REST's guess-file path (density dataset "init_guess") with the converged density
+    from rest_amd import h5
+    h5.write_init_guess(tmp_path / "guess.h5", [out.dm])
+    out3, t = rs.run_molecule(m, guess="file", guess_path=tmp_path / "guess.h5")
+    say(f"  guess-file start: {out3.iterations} iterations, E(D0) {t['e_guess_d0']:.10f}")
+    assert out3.converged and out3.iterations <= 2 and abs(out3.energy - out.energy) < 1e-9
