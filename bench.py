@@ -34,7 +34,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3}   # MI355X dense matrix peaks of the instruction used
+MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}   # MI355X dense matrix peaks per instruction type
+# MFMA products per algorithmic product on the split-BF16 planes (a + b < S)
+BF16_PRODUCTS = {"bf16x2": 3, "bf16x3": 6}
 
 PHASE_KERNEL = {"j_gamma": "k_jgamma", "j_acc": "k_jacc", "k_half": "k_khalf", "k_syrk": "k_ksyrk"}
 
@@ -48,8 +50,11 @@ def parse():
     ap.add_argument("--clusters", type=int, default=512, help="batched mode: number of water clusters")
     ap.add_argument("--workload", default="h2o64", choices=["h2o64", "h2o16"])
     ap.add_argument("--model", default=str(ROOT / "rest_amd" / "models" / "learned_guess_h2o.npz"))
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"],
-                    help="fp64: plain FP64 path; fp32: FP32 copy of B, FP32 MFMA K, FP64 accumulation of J")
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x2"],
+                    help="jk mode: fp64 plain path; fp32: FP32 copy of B, FP32 MFMA K, FP64 accumulation of J; "
+                         "bf16x3 / bf16x2: K on three / two BF16 planes of B (6 / 3 MFMA products), J on the FP32 copy")
+    ap.add_argument("--mixed-prec", default="bf16x3", choices=["fp32", "bf16x3"],
+                    help="scf mode: J/K precision of the mixed SCF before its switch to FP64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -115,8 +120,10 @@ def main():
     return main_jk(args)
 
 
-def roofline_from_detail(detail: dict):
-    """Dominant kernel of the timed region and its achieved rate from the library's per-launch work."""
+def roofline_from_detail(detail: dict, bf16_products: int = 6):
+    """Dominant kernel of the timed region and its achieved rate from the library's per-launch work.
+    BF16-plane launches execute bf16_products MFMA products per algorithmic product: their rate is
+    the executed MFMA rate against the BF16 peak."""
     best = None
     for (prec, phase), (ms, n, work) in detail.items():
         if phase in PHASE_KERNEL and n and (best is None or ms > best[2]):
@@ -126,8 +133,12 @@ def roofline_from_detail(detail: dict):
     if phase.startswith("j_"):
         ach, peak, unit, bound = work / sec / 1e9, HBM_PEAK_GBS, "GB/s", "hbm"
     else:
-        ach, peak, unit, bound = work / sec / 1e12, MFMA_PEAK_TFS[prec], "TFLOP/s", "mfma"
-    kernel = PHASE_KERNEL[phase] + ("<double>" if prec == "fp64" else "<float>")
+        mult = bf16_products if prec == "bf16" else 1
+        ach, peak, unit, bound = mult * work / sec / 1e12, MFMA_PEAK_TFS[prec], "TFLOP/s", "mfma"
+    if prec == "bf16" and not phase.startswith("j_"):
+        kernel = {"k_half": "k_bf_khalf", "k_syrk": "k_bf_ksyrk"}[phase] + f"<{3 if bf16_products == 6 else 2}>"
+    else:
+        kernel = PHASE_KERNEL[phase] + ("<double>" if prec == "fp64" else "<float>")
     return {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
             "traffic": load_traffic(f"{PHASE_KERNEL[phase]}_{prec}_scf"), "kernel": kernel,
             "launches": n, "avg_launch_ms": ms / n, "algorithmic_per_launch": work / n}
@@ -156,7 +167,9 @@ def main_scf(args):
     stream = torch.cuda.Stream(device=torch.device("cuda:0"))
     torch.cuda.set_stream(stream)
     r.set_stream(stream.cuda_stream)
-    binfo = mol.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32)
+    mixed_prec = jk.PREC_NAMES[args.mixed_prec]
+    store = jk.STORE_FP64 | jk.STORE_FP32 | (jk.STORE_BF16 if args.mixed_prec.startswith("bf16") else 0)
+    binfo = mol.build_rimatr(r, 1e-8, store)
     t2 = time.perf_counter()
     net = lg.Net.load(model)
     net.set_stream(stream.cuda_stream)
@@ -170,8 +183,8 @@ def main_scf(args):
         return out, {"guess": "sad", "e_guess": e0}
 
     def mixed():
-        g = lg.learned_guess(net, mol, r, S, H, jk.PREC_FP32)
-        out = rs.rhf(r, S, H, enuc, mol.nelectron, None, rs.default_ctrl(jk_prec=jk.PREC_FP32), c_guess=g.c1)
+        g = lg.learned_guess(net, mol, r, S, H, mixed_prec)
+        out = rs.rhf(r, S, H, enuc, mol.nelectron, None, rs.default_ctrl(jk_prec=mixed_prec), c_guess=g.c1)
         return out, {"guess": g.kind, "e_learned": g.e_learned, "e_sad": g.e_sad, "guess_seconds": g.seconds}
 
     for _ in range(args.warmup):
@@ -199,7 +212,7 @@ def main_scf(args):
     r.profile_enable(False)
     op, ip, om, im = runs[-1]
     t_plain, t_mixed = float(np.median(tp)), float(np.median(tm))
-    roofline = roofline_from_detail(detail_prof)
+    roofline = roofline_from_detail(detail_prof, BF16_PRODUCTS.get(args.mixed_prec, 6))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -225,11 +238,14 @@ def main_scf(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (plain); f32->f64 J/K, f64 elsewhere (mixed)",
+        "dtype": "f64 (plain); " + ("f32" if args.mixed_prec == "fp32" else "split-bf16 K (3 planes) + f32 J")
+                 + "->f64 J/K, f64 elsewhere (mixed)",
         "data": "synthetic (seeded water cluster geometry; integrals from this build's engine)",
         "config": {"workload": f"(H2O){n} seed {seed}, def2-SVP / def2-SVP-JKFIT, nbf={mol.nbf} naux={mol.naux} "
                                f"nocc={mol.nelectron // 2}; RI-HF to |dE| <= 1e-8 Eh, ||dD||/nbf <= 1e-5, "
-                               "||d eps|| <= 1e-5 (REST defaults); FP64+FP32 B resident in HBM",
+                               "||d eps|| <= 1e-5 (REST defaults); FP64+FP32"
+                               + ("+BF16-plane" if store & jk.STORE_BF16 else "") + " B resident in HBM",
+                   "mixed_prec": args.mixed_prec,
                    "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nelectron // 2},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -350,9 +366,11 @@ def main_jk(args):
 
     c = wl.random_orthonormal(seed, nbf, nocc)
     d = c @ c.T
-    prec = jk.PREC_FP64 if args.prec == "fp64" else jk.PREC_FP32
+    prec = jk.PREC_NAMES[args.prec]
+    bf = args.prec.startswith("bf16")
     ri = jk.DeviceRI(nbf, naux)
-    ri.fill_synthetic(seed, jk.STORE_FP64 | (jk.STORE_FP32 if prec == jk.PREC_FP32 else 0))
+    ri.fill_synthetic(seed, jk.STORE_FP64 | (jk.STORE_FP32 if prec != jk.PREC_FP64 else 0)
+                      | (jk.STORE_BF16 if bf else 0))
     # a non-default torch stream (default priority) shared by the library and
     # the timing events: the null stream would hand the library back its own
     stream = torch.cuda.Stream(device=dev)
@@ -417,13 +435,17 @@ def main_jk(args):
     pk = per_kernel[dom]
     per_launch = amount / pk["launches_per_step"]
     sec = pk["avg_launch_ms"] / 1e3
+    kpeak = MFMA_PEAK_TFS["bf16" if bf else args.prec]
+    kmult = BF16_PRODUCTS.get(args.prec, 1)       # executed MFMA products per algorithmic product
     if bound == "hbm":
         ach, peak, unit = per_launch / sec / 1e9, HBM_PEAK_GBS, "GB/s"
     else:
-        ach, peak, unit = per_launch / sec / 1e12, MFMA_PEAK_TFS[args.prec], "TFLOP/s"
+        ach, peak, unit = kmult * per_launch / sec / 1e12, kpeak, "TFLOP/s"
+    kname = PHASE_KERNEL[dom]
+    if bf and dom.startswith("k_"):
+        kname = kname.replace("k_", "k_bf_", 1) + f"<{3 if args.prec == 'bf16x3' else 2}>"
     roofline = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
-                "traffic": load_traffic(PHASE_KERNEL[dom]), "kernel": PHASE_KERNEL[dom],
-                "algorithmic_per_launch": per_launch}
+                "traffic": load_traffic(kname), "kernel": kname, "algorithmic_per_launch": per_launch}
     # secondary figures: J against HBM, K (both stages) against FP64 MFMA
     j_ms = sum(per_kernel.get(k, {}).get("ms_per_step", 0.0) for k in ("j_gamma", "j_acc"))
     k_ms = sum(per_kernel.get(k, {}).get("ms_per_step", 0.0) for k in ("k_half", "k_syrk"))
@@ -431,7 +453,7 @@ def main_jk(args):
         "j_build_hbm_GBps": jbytes / (j_ms / 1e3) / 1e9 if j_ms else None,
         "j_build_hbm_frac": (jbytes / (j_ms / 1e3) / 1e9) / HBM_PEAK_GBS if j_ms else None,
         "k_build_TFLOPs": kf["total"] / (k_ms / 1e3) / 1e12 if k_ms else None,
-        "k_build_mfma_frac": (kf["total"] / (k_ms / 1e3) / 1e12) / MFMA_PEAK_TFS[args.prec] if k_ms else None,
+        "k_build_mfma_frac": (kmult * kf["total"] / (k_ms / 1e3) / 1e12) / kpeak if k_ms else None,
         "per_kernel": per_kernel,
     }
 
@@ -441,7 +463,8 @@ def main_jk(args):
 
     out = {
         "metric": f"RI-V J/K builds/s (one J + one K), {shape.name} shape, "
-                  f"{'plain FP64' if args.prec == 'fp64' else 'FP32 B + FP32 MFMA'}",
+                  + {"fp64": "plain FP64", "fp32": "FP32 B + FP32 MFMA", "bf16x3": "K on 3 BF16 planes, FP32 J",
+                     "bf16x2": "K on 2 BF16 planes, FP32 J"}[args.prec],
         "value": args.steps / (ms_total / 1e3),
         "unit": "J/K builds/s",
         "n_gpus": 1,
@@ -452,13 +475,11 @@ def main_jk(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.prec == "fp64" else "f32",
+        "dtype": {"fp64": "f64", "fp32": "f32"}.get(args.prec, "split-bf16 K, f32 J"),
         "data": "synthetic",
         "config": {"workload": f"{shape.name}: nbf={nbf} naux={naux} npair={shape.npair} nocc={nocc}, "
                                f"packed FP64 B resident in HBM (seed {seed}), D=C C^T, K occupied route",
                    "nbf": nbf, "naux": naux, "nocc": nocc, "nspin": 1},
-        "north_star_metric": "SCF time to 1e-8 Eh, (H2O)64 def2-SVP: not yet measurable (needs the B producer "
-                             "and the SCF driver); this line measures the per-iteration J/K hot path",
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": detail,

@@ -51,16 +51,22 @@ enum {
 
 /* Arithmetic used by a J/K call. */
 enum {
-    RJK_PREC_FP64 = 0,   /* FP64 loads, FP64 MFMA / FMA, FP64 accumulation   */
-    RJK_PREC_FP32 = 1,   /* FP32 copy of B; J: FP64 accumulation             */
-    RJK_PREC_BF16X2 = 2  /* split hi/lo BF16 copy of B (reserved)            */
+    RJK_PREC_FP64 = 0,   /* FP64 loads, FP64 MFMA / FMA, FP64 accumulation          */
+    RJK_PREC_FP32 = 1,   /* FP32 copy of B; J: FP64 accumulation                    */
+    RJK_PREC_BF16X2 = 2, /* K: two BF16 planes per operand, 3 products (~2^-16)     */
+    RJK_PREC_BF16X3 = 3  /* K: three BF16 planes per operand, 6 products (~2^-24)   */
+    /* The BF16 precisions run K on BF16 MFMA with FP32 accumulation and FP64
+     * split sums; J then runs on the FP32 copy of B when it is stored, else on
+     * the FP64 copy. */
 };
 
 /* Which device copies of B to keep (bit mask for rjk_set_rimatr*). */
 enum {
     RJK_STORE_FP64 = 1,
     RJK_STORE_FP32 = 2,
-    RJK_STORE_BF16X2 = 4
+    RJK_STORE_BF16 = 4   /* three BF16 planes of every B_P as a full padded
+                          * square: 6·naux·nbf_pad² bytes, nbf_pad = nbf
+                          * rounded up to 128 (serves both BF16 precisions) */
 };
 
 /* ---- context ------------------------------------------------------------ */
@@ -164,8 +170,8 @@ int rjk_vk_upper_with_ri_v_use_dm_only_sync(rjk_ctx* ctx, const double* dm,
  * launches[5] and resets the counters. */
 int rjk_profile_enable(rjk_ctx* ctx, int on);
 int rjk_profile_read(rjk_ctx* ctx, double* ms, uint64_t* launches);
-/* The same counters split by arithmetic, [2][5] row-major (row 0 FP64, row 1
- * FP32), plus the algorithmic work of those launches: bytes for the J
+/* The same counters split by arithmetic, [3][5] row-major (row 0 FP64, row 1
+ * FP32, row 2 BF16 planes), plus the algorithmic work of those launches: bytes for the J
  * phases (npair·naux·s_B + 8·nspin·npair per pass), FLOP for the K phases
  * (2·nbf²·nk·P per half-transform, nbf(nbf+1)·nk·P per SYRK), 0 for phase 4.
  * Resets the counters like rjk_profile_read. */

@@ -11,8 +11,9 @@
  * ‖ΔD‖_F/nbf and ‖Δε‖₂ meet the thresholds (check_scf_convergence, :1875-1951).
  *
  * All matrices stay in HBM; J/K use the context's resident B.  With
- * jk_prec = RJK_PREC_FP32 the J/K builds run on the FP32 copy of B until the
- * DIIS error max|FDS − SDF| falls below switch_diis_err, then switch to FP64
+ * jk_prec = RJK_PREC_FP32 (or RJK_PREC_BF16X3 / RJK_PREC_BF16X2: K on the
+ * BF16 planes of B, J on its FP32 copy) the J/K builds run at that precision
+ * until the DIIS error max|FDS − SDF| falls below switch_diis_err, then switch to FP64
  * for the remaining iterations (the FP64 builds act as iterative
  * refinement) and restart DIIS from the first FP64 Fock matrix; convergence
  * is only declared on two consecutive FP64 energies.
@@ -38,10 +39,18 @@ typedef struct rscf_ctrl {
     double scf_acc_rho;      /* REST scf_acc_rho (1e-5)                    */
     double scf_acc_eev;      /* REST scf_acc_eev (1e-5)                    */
     double scf_acc_etot;     /* REST scf_acc_etot (1e-8)                   */
-    int jk_prec;             /* RJK_PREC_FP64 (plain) or RJK_PREC_FP32 (mixed) */
+    int jk_prec;             /* RJK_PREC_FP64 (plain) or FP32 / BF16X3 / BF16X2 (mixed) */
     double switch_diis_err;  /* mixed mode: switch to FP64 below this max|e| (1e-5) */
     int print_level;         /* 0 silent, 1 one line per iteration on stdout */
+    int switch_mode;         /* mixed mode, at the switch: RSCF_SWITCH_SHIFT (default) or _RESTART */
 } rscf_ctrl;
+
+/* What DIIS keeps at the precision switch.  RESTART: the stored low-precision
+ * vectors are dropped.  SHIFT: the first FP64 iteration also builds F at the
+ * low precision for the same density and adds δ = F(FP64) − F(low) to every
+ * stored Fock matrix (error vectors: + δ D_k S − S D_k δ), so the history
+ * continues at FP64 quality for the cost of one low-precision build. */
+enum { RSCF_SWITCH_RESTART = 0, RSCF_SWITCH_SHIFT = 1 };
 
 /* Per-iteration record fields (iter_log rows of RSCF_LOG_FIELDS doubles). */
 enum { RSCF_LOG_E = 0, RSCF_LOG_DE, RSCF_LOG_DIIS_MAX, RSCF_LOG_DM, RSCF_LOG_EEV, RSCF_LOG_PREC,
@@ -85,8 +94,8 @@ int rscf_uhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t
  * as "Initial guess HF energy"), F C = S C ε and the density D1 of those
  * orbitals, which is where REST's scf() starts.  nspin 1: nelec[0] is the
  * electron count, d0 and d1 are nbf²; nspin 2: nelec = {nα, nβ}, d0 and d1
- * are 2×nbf².  prec: J/K arithmetic (RJK_PREC_FP64 as in REST, or
- * RJK_PREC_FP32 to rank guesses cheaply).  mo_coeff / mo_energy may be NULL.
+ * are 2×nbf².  prec: J/K arithmetic (RJK_PREC_FP64 as in REST, or a lower
+ * precision to rank guesses cheaply).  mo_coeff / mo_energy may be NULL.
  * energy_d1 (may be NULL) receives E(D1), the energy REST prints as "The
  * total energy ... by the initial guess" at the start of scf() (:3195-3199),
  * from the occupied-orbital J/K of the new orbitals. */

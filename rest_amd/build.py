@@ -24,7 +24,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-I", str(PKG.parent / "include")]
 SOURCES = ["rjk_api.cpp", "rjk_j.hip", "rjk_k.hip", "rjk_util.hip", "rsi_int.cpp", "rsi_gpu3c.hip", "rsi_build.hip",
-           "rscf.hip", "rlg.hip", "rjk_batch.hip"]
+           "rscf.hip", "rlg.hip", "rjk_batch.hip", "rjk_kbf.hip"]
 # MFMA accumulators in VGPRs: the K tiles carry 64-128 accumulator registers
 # across a software-pipelined loop; the default AGPR form makes the allocator
 # copy them VGPR<->AGPR every iteration.

@@ -63,7 +63,7 @@ PhaseScope::PhaseScope(rjk_ctx* c, int phase, int prec, double work) : ctx(c), o
         (void)hipEventCreate(&ev.b);
     }
     ev.phase = phase;
-    ev.prec = prec == RJK_PREC_FP64 ? 0 : 1;
+    ev.prec = prof_row(prec);
     ev.work = work;
     (void)hipEventRecord(ev.a, ctx->stream);
 }
@@ -112,6 +112,7 @@ extern "C" int rjk_destroy(rjk_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->b64) (void)hipFree(ctx->b64);
     if (ctx->b32) (void)hipFree(ctx->b32);
+    if (ctx->bpl) (void)hipFree(ctx->bpl);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->solver) rocblas_destroy_handle((rocblas_handle)ctx->solver);
     for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -157,7 +158,8 @@ namespace rjk {
 int alloc_b(rjk_ctx* ctx, unsigned mask) {
     if (!(mask & RJK_STORE_FP64))
         return fail(ctx, RJK_ERR_ARG, "store_mask must include RJK_STORE_FP64 (the FP64 copy is the reference)");
-    if (mask & RJK_STORE_BF16X2) return fail(ctx, RJK_ERR_ARG, "RJK_STORE_BF16X2 is not available in this build");
+    if (mask & ~unsigned(RJK_STORE_FP64 | RJK_STORE_FP32 | RJK_STORE_BF16))
+        return fail(ctx, RJK_ERR_ARG, "store_mask holds unknown bits");
     RJK_HIP(ctx, hipSetDevice(ctx->device));
     RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const size_t n = ctx->naux * ctx->ldb;
@@ -179,11 +181,28 @@ int alloc_b(rjk_ctx* ctx, unsigned mask) {
         (void)hipFree(ctx->b32);
         ctx->b32 = nullptr;
     }
+    if ((mask & RJK_STORE_BF16) && !ctx->bpl) {
+        const size_t pad = round_up(ctx->nbf, kTileMN);
+        const size_t bytes = (size_t)kBfStoredPlanes * ctx->naux * pad * pad * 2;
+        if (hipMalloc(&ctx->bpl, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->bpl = nullptr;
+            return fail(ctx, RJK_ERR_NOMEM, "B (BF16 planes) allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        ctx->bpl_pad = pad;
+    }
+    if (!(mask & RJK_STORE_BF16) && ctx->bpl) {
+        (void)hipFree(ctx->bpl);
+        ctx->bpl = nullptr;
+        ctx->bpl_pad = 0;
+    }
     return RJK_OK;
 }
 
 int finish_b(rjk_ctx* ctx) {
     if (ctx->b32) RJK_HIP(ctx, launch_to_fp32(ctx->stream, ctx->b64, ctx->naux * ctx->ldb, ctx->b32));
+    if (ctx->bpl)
+        RJK_HIP(ctx, launch_bf_planes_b(ctx->stream, ctx->b64, ctx->ldb, ctx->nbf, ctx->bpl_pad, ctx->naux, ctx->bpl));
     RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return RJK_OK;
 }
@@ -256,15 +275,30 @@ extern "C" int rjk_get_rimatr(rjk_ctx* ctx, double* rimatr) {
 // ---------------------------------------------------------------------------
 namespace {
 
-int check_common(rjk_ctx* ctx, size_t spin_channel, int prec, const void** bptr) {
+// The B copy a call reads.  J: the BF16 precisions run on the FP32 copy when it is
+// stored, else on the FP64 copy (*jprec receives the precision J runs at).  K: the
+// BF16 precisions need the planes.
+int check_common(rjk_ctx* ctx, size_t spin_channel, int prec, bool for_k, const void** bptr, int* jprec = nullptr) {
     if (spin_channel != 1 && spin_channel != 2)
         return fail(ctx, RJK_ERR_ARG, "spin_channel must be 1 or 2, got " + std::to_string(spin_channel));
     if (!ctx->b64) return fail(ctx, RJK_ERR_STATE, "B is not set (rjk_set_rimatr / rjk_set_ri3fn first)");
-    if (prec == RJK_PREC_FP64) *bptr = ctx->b64;
-    else if (prec == RJK_PREC_FP32) {
+    if (!is_bf16(prec) && prec != RJK_PREC_FP64 && prec != RJK_PREC_FP32)
+        return fail(ctx, RJK_ERR_ARG, "unsupported precision " + std::to_string(prec));
+    int eff = prec;
+    if (is_bf16(prec)) {
+        if (for_k) {
+            if (!ctx->bpl) return fail(ctx, RJK_ERR_STATE, "BF16 planes of B not stored (RJK_STORE_BF16)");
+            *bptr = ctx->bpl;
+            return RJK_OK;
+        }
+        eff = ctx->b32 ? RJK_PREC_FP32 : RJK_PREC_FP64;
+    }
+    if (jprec) *jprec = eff;
+    if (eff == RJK_PREC_FP64) *bptr = ctx->b64;
+    else {
         if (!ctx->b32) return fail(ctx, RJK_ERR_STATE, "FP32 copy of B not stored (RJK_STORE_FP32)");
         *bptr = ctx->b32;
-    } else return fail(ctx, RJK_ERR_ARG, "unsupported precision " + std::to_string(prec));
+    }
     return RJK_OK;
 }
 
@@ -275,7 +309,7 @@ extern "C" int rjk_vj_upper_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_
     if (!ctx) return RJK_ERR_ARG;
     if (!dm_dev || !vj_dev) return fail(ctx, RJK_ERR_ARG, "vj: NULL device pointer");
     const void* b = nullptr;
-    int rc = check_common(ctx, spin_channel, prec, &b);
+    int rc = check_common(ctx, spin_channel, prec, false, &b, &prec);
     if (rc) return rc;
     RJK_HIP(ctx, hipSetDevice(ctx->device));
     const int ns = (int)spin_channel;
@@ -376,11 +410,12 @@ extern "C" int rjk_vj_upper_with_ri_v_sync(rjk_ctx* ctx, const double* dm, size_
 // ---------------------------------------------------------------------------
 namespace {
 
-void k_plan_cols(KPlan& kp, size_t nbf, const size_t* ncol, int ns) {
+void k_plan_cols(KPlan& kp, size_t nbf, const size_t* ncol, int ns, int prec) {
+    const size_t gran = is_bf16(prec) ? kBfColGran : kTileK;
     kp.nbf_pad = round_up(nbf, kTileMN);
-    kp.col_cnt[0] = round_up(ncol[0], kTileK);
+    kp.col_cnt[0] = round_up(ncol[0], gran);
     kp.col_off[0] = 0;
-    kp.col_cnt[1] = ns == 2 ? round_up(ncol[1], kTileK) : 0;
+    kp.col_cnt[1] = ns == 2 ? round_up(ncol[1], gran) : 0;
     kp.col_off[1] = kp.col_cnt[0];
     kp.nk = round_up(kp.col_cnt[0] + kp.col_cnt[1], kColPad);
 }
@@ -405,7 +440,7 @@ int k_plan_ws(rjk_ctx* ctx, KPlan& kp, int ns, int prec, void** cocc, void** sig
     size_t fixed = small + nsplit * slot_bytes;
     size_t avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
     size_t chunk = perP ? std::min(ctx->naux, std::max<size_t>(1, avail / perP)) : ctx->naux;
-    if (prec == RJK_PREC_FP32 && kp.nk > 0) {
+    if (prec != RJK_PREC_FP64 && kp.nk > 0) {
         // FP32 accumulators: each split sums at most kFp32Chain products (pps auxiliary
         // columns × nk) before its partial joins the FP64 sum, so rounding does not grow
         // with naux.  Up to half the workspace goes to the partials.
@@ -441,16 +476,19 @@ int k_run(rjk_ctx* ctx, const KPlan& kp, int ns, int prec, const void* b, const 
         RJK_HIP(ctx, hipMemsetAsync(vk_dev, 0, (size_t)ns * ctx->npair * sizeof(double), st));
         return RJK_OK;
     }
+    const bool bf = is_bf16(prec);
     for (size_t P0 = 0; P0 < ctx->naux; P0 += kp.chunk) {
         size_t Pc = std::min(kp.chunk, ctx->naux - P0);
         const double nb = (double)ctx->nbf;
         {
             PhaseScope ps(ctx, 2, prec, 2.0 * nb * nb * (double)kp.nk * (double)Pc);
-            RJK_HIP(ctx, launch_khalf(st, prec, b, ctx->ldb, ctx->nbf, P0, Pc, kp, cocc, m));
+            if (bf) RJK_HIP(ctx, launch_bf_khalf(st, prec, b, ctx->naux, ctx->nbf, P0, Pc, kp, cocc, m));
+            else RJK_HIP(ctx, launch_khalf(st, prec, b, ctx->ldb, ctx->nbf, P0, Pc, kp, cocc, m));
         }
         {
             PhaseScope ps(ctx, 3, prec, nb * (nb + 1.0) * (double)kp.nk * (double)Pc);
-            RJK_HIP(ctx, launch_ksyrk(st, prec, m, Pc, kp, ns, sign, P0 > 0 ? 1 : 0, kpart));
+            if (bf) RJK_HIP(ctx, launch_bf_ksyrk(st, prec, m, Pc, kp, ns, sign, P0 > 0 ? 1 : 0, kpart));
+            else RJK_HIP(ctx, launch_ksyrk(st, prec, m, Pc, kp, ns, sign, P0 > 0 ? 1 : 0, kpart));
         }
     }
     PhaseScope ps(ctx, 4);
@@ -466,7 +504,7 @@ extern "C" int rjk_vk_upper_occ_dev(rjk_ctx* ctx, const double* eigv_dev, size_t
     if (!ctx) return RJK_ERR_ARG;
     if (!eigv_dev || !homo || !occ_dev || !vk_dev) return fail(ctx, RJK_ERR_ARG, "vk: NULL pointer");
     const void* b = nullptr;
-    int rc = check_common(ctx, spin_channel, prec, &b);
+    int rc = check_common(ctx, spin_channel, prec, true, &b);
     if (rc) return rc;
     const int ns = (int)spin_channel;
     size_t nw[2] = {0, 0};
@@ -477,15 +515,19 @@ extern "C" int rjk_vk_upper_occ_dev(rjk_ctx* ctx, const double* eigv_dev, size_t
     }
     RJK_HIP(ctx, hipSetDevice(ctx->device));
     KPlan kp;
-    k_plan_cols(kp, ctx->nbf, nw, ns);
+    k_plan_cols(kp, ctx->nbf, nw, ns, prec);
     void *cocc, *sign, *m;
     double* kpart;
     rc = k_plan_ws(ctx, kp, ns, prec, &cocc, &sign, &kpart, &m);
     if (rc) return rc;
     {
         PhaseScope ps(ctx, 4);
-        RJK_HIP(ctx, launch_prep_cocc(ctx->stream, prec, eigv_dev, ctx->nbf, num_state, nw, occ_dev, ns, kp, cocc,
-                                      sign));
+        if (is_bf16(prec))
+            RJK_HIP(ctx, launch_bf_prep_c(ctx->stream, prec, eigv_dev, ctx->nbf, num_state, nw, occ_dev, ns, kp,
+                                          cocc, sign));
+        else
+            RJK_HIP(ctx, launch_prep_cocc(ctx->stream, prec, eigv_dev, ctx->nbf, num_state, nw, occ_dev, ns, kp,
+                                          cocc, sign));
     }
     return k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, vk_dev);
 }
@@ -537,7 +579,7 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
     if (!ctx) return RJK_ERR_ARG;
     if (!dm || !vk_upper) return fail(ctx, RJK_ERR_ARG, "vk: NULL host pointer");
     const void* b = nullptr;
-    int rc = check_common(ctx, spin_channel, prec, &b);
+    int rc = check_common(ctx, spin_channel, prec, true, &b);
     if (rc) return rc;
     const size_t nbf = ctx->nbf, n2 = nbf * nbf;
     const int ns = (int)spin_channel;
@@ -599,7 +641,7 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
         ncol[s] = sel_idx[s].size();
     }
     KPlan kp;
-    k_plan_cols(kp, nbf, ncol, ns);
+    k_plan_cols(kp, nbf, ncol, ns, prec);
     void *cocc, *sign, *m;
     double* kpart;
     rc = k_plan_ws(ctx, kp, ns, prec, &cocc, &sign, &kpart, &m);
@@ -613,8 +655,12 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
                                     st));
         RJK_HIP(ctx, hipMemcpyAsync(d_sel + s * nbf, sel_lam[s].data(), ncol[s] * sizeof(double),
                                     hipMemcpyHostToDevice, st));
-        RJK_HIP(ctx, launch_prep_cocc_idx(st, prec, d_vec + s * n2, nbf, d_idx + s * nbf, d_sel + s * nbf, ncol[s],
-                                          kp.col_off[s], kp, cocc, sign));
+        if (is_bf16(prec))
+            RJK_HIP(ctx, launch_bf_prep_c_idx(st, prec, d_vec + s * n2, nbf, d_idx + s * nbf, d_sel + s * nbf,
+                                              ncol[s], kp.col_off[s], kp, cocc, sign));
+        else
+            RJK_HIP(ctx, launch_prep_cocc_idx(st, prec, d_vec + s * n2, nbf, d_idx + s * nbf, d_sel + s * nbf,
+                                              ncol[s], kp.col_off[s], kp, cocc, sign));
     }
     rc = k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, d_out);
     if (rc) return rc;
@@ -660,7 +706,7 @@ int profile_collect(rjk_ctx* ctx) {
     return RJK_OK;
 }
 void profile_reset(rjk_ctx* ctx) {
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < kNumPrec; ++p)
         for (int i = 0; i < kNumPhases; ++i) {
             ctx->prof_ms[p][i] = 0;
             ctx->prof_n[p][i] = 0;
@@ -674,8 +720,12 @@ extern "C" int rjk_profile_read(rjk_ctx* ctx, double* ms, uint64_t* launches) {
     int rc = profile_collect(ctx);
     if (rc) return rc;
     for (int i = 0; i < kNumPhases; ++i) {
-        if (ms) ms[i] = ctx->prof_ms[0][i] + ctx->prof_ms[1][i];
-        if (launches) launches[i] = ctx->prof_n[0][i] + ctx->prof_n[1][i];
+        if (ms) ms[i] = 0;
+        if (launches) launches[i] = 0;
+        for (int p = 0; p < kNumPrec; ++p) {
+            if (ms) ms[i] += ctx->prof_ms[p][i];
+            if (launches) launches[i] += ctx->prof_n[p][i];
+        }
     }
     profile_reset(ctx);
     return RJK_OK;
@@ -685,7 +735,7 @@ extern "C" int rjk_profile_read_detail(rjk_ctx* ctx, double* ms, uint64_t* launc
     if (!ctx) return RJK_ERR_ARG;
     int rc = profile_collect(ctx);
     if (rc) return rc;
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < kNumPrec; ++p)
         for (int i = 0; i < kNumPhases; ++i) {
             if (ms) ms[p * kNumPhases + i] = ctx->prof_ms[p][i];
             if (launches) launches[p * kNumPhases + i] = ctx->prof_n[p][i];

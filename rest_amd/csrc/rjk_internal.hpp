@@ -19,6 +19,12 @@ constexpr int kNumPhases = 5;        // J-gamma, J-acc, K-half, K-syrk, K-other
 constexpr int kTileMN = 128;         // nbf padding for the K tiles
 constexpr int kTileK = 16;           // k-block of the K tiles
 constexpr int kColPad = 64;          // occupied-column padding of M (all spins)
+constexpr int kBfColGran = 32;       // BF16 K: per-spin column granule (one MFMA k-block)
+constexpr int kBfStoredPlanes = 3;   // BF16 planes kept per B element (RJK_STORE_BF16)
+constexpr int kNumPrec = 3;          // profile rows: FP64, FP32, BF16 planes
+
+inline bool is_bf16(int prec) { return prec == RJK_PREC_BF16X2 || prec == RJK_PREC_BF16X3; }
+inline int prof_row(int prec) { return prec == RJK_PREC_FP64 ? 0 : (is_bf16(prec) ? 2 : 1); }
 
 inline size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -36,6 +42,8 @@ struct rjk_ctx {
     hipStream_t stream = nullptr;
     double* b64 = nullptr;    // [naux][ldb]
     float* b32 = nullptr;     // [naux][ldb]
+    void* bpl = nullptr;      // BF16 planes [kBfStoredPlanes][naux][nbf_pad][nbf_pad]
+    size_t bpl_pad = 0;       // nbf_pad of the planes
     size_t ws_limit = size_t(4) << 30;
     void* ws = nullptr;       // grow-only workspace
     size_t ws_size = 0;
@@ -43,9 +51,9 @@ struct rjk_ctx {
     std::string err;
     // phase timing
     bool prof = false;
-    double prof_ms[2][rjk::kNumPhases] = {};     // [FP64, FP32][phase]
-    uint64_t prof_n[2][rjk::kNumPhases] = {};
-    double prof_work[2][rjk::kNumPhases] = {};   // algorithmic bytes (J phases) or FLOP (K phases)
+    double prof_ms[rjk::kNumPrec][rjk::kNumPhases] = {};     // [FP64, FP32, BF16][phase]
+    uint64_t prof_n[rjk::kNumPrec][rjk::kNumPhases] = {};
+    double prof_work[rjk::kNumPrec][rjk::kNumPhases] = {};   // algorithmic bytes (J) or FLOP (K)
     struct EvPair { hipEvent_t a, b; int phase; int prec; double work; };
     std::vector<EvPair> ev_pending;
     std::vector<EvPair> ev_free;
@@ -100,7 +108,7 @@ struct KPlan {
     size_t chunk = 0;        // P per chunk
     size_t nsplit = 0;       // split-K slots of the SYRK
 };
-size_t k_elem_size(int prec);   // element size of C', M' and signs for a K precision
+size_t k_elem_size(int prec);   // bytes per element of C', M' and signs (all planes) for a K precision
 hipError_t launch_prep_cocc(hipStream_t st, int prec, const double* eigv, size_t nbf, size_t nstate,
                             const size_t* nw, const double* occ, int nspin, const KPlan& kp, void* cocc,
                             void* sign);
@@ -113,6 +121,21 @@ hipError_t launch_ksyrk(hipStream_t st, int prec, const void* m, size_t Pc, cons
                         const void* sign, int beta, double* kpart);
 hipError_t launch_kreduce(hipStream_t st, const double* kpart, const KPlan& kp, int nspin, size_t nbf,
                           double scale, double* vk);
+
+// K on BF16 planes (rjk_kbf.hip): C' planes [S][nk][nbf_pad], sign masks [nk] (uint16),
+// M' planes [S][Pc][nbf_pad][nk]
+int bf_planes(int prec);
+hipError_t launch_bf_planes_b(hipStream_t st, const double* b64, size_t ldb, size_t nbf, size_t nbf_pad,
+                              size_t naux, void* planes);
+hipError_t launch_bf_prep_c(hipStream_t st, int prec, const double* eigv, size_t nbf, size_t nstate,
+                            const size_t* nw, const double* occ, int nspin, const KPlan& kp, void* ct, void* smask);
+hipError_t launch_bf_prep_c_idx(hipStream_t st, int prec, const double* vecs, size_t nbf, const int* idx,
+                                const double* lam, size_t ncol, size_t col_off, const KPlan& kp, void* ct,
+                                void* smask);
+hipError_t launch_bf_khalf(hipStream_t st, int prec, const void* bpl, size_t naux, size_t nbf, size_t P0, size_t Pc,
+                           const KPlan& kp, const void* ct, void* mpl);
+hipError_t launch_bf_ksyrk(hipStream_t st, int prec, const void* mpl, size_t Pc, const KPlan& kp, int nspin,
+                           const void* smask, int beta, double* kpart);
 
 // util
 hipError_t launch_fill_synth(hipStream_t st, uint64_t seed, size_t npair, size_t naux, size_t ldb, double* b);

@@ -153,7 +153,10 @@ hipError_t ksyrk_t(hipStream_t st, const T* m, size_t Pc, const KPlan& kp, int n
 
 }  // namespace
 
-size_t k_elem_size(int prec) { return prec == RJK_PREC_FP64 ? sizeof(double) : sizeof(float); }
+size_t k_elem_size(int prec) {
+    if (is_bf16(prec)) return 2 * (size_t)bf_planes(prec);
+    return prec == RJK_PREC_FP64 ? sizeof(double) : sizeof(float);
+}
 
 hipError_t launch_prep_cocc(hipStream_t st, int prec, const double* eigv, size_t nbf, size_t nstate,
                             const size_t* nw, const double* occ, int nspin, const KPlan& kp, void* cocc,

@@ -85,6 +85,26 @@ __global__ void k_commutator(const double* __restrict__ T, size_t n, double* __r
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// e += T − Tᵀ (n × n, column-major)
+__global__ void k_commutator_acc(const double* __restrict__ T, size_t n, double* __restrict__ e) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = t % n, j = t / n;
+        e[t] += T[t] - T[i * n + j];
+    }
+}
+
+// y += a·x
+__global__ void k_axpy_inplace(double* __restrict__ y, const double* __restrict__ x, double a, size_t n2) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
+        y[t] += a * x[t];
+}
+
+// y = x − y
+__global__ void k_delta_inplace(const double* __restrict__ x, double* __restrict__ y, size_t n2) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
+        y[t] = x[t] - y[t];
+}
+
 // partial[b] = Σ (A − B)²
 __global__ void k_sumsq_diff(const double* __restrict__ A, const double* __restrict__ B, size_t n2,
                              double* __restrict__ partial) {
@@ -227,6 +247,7 @@ extern "C" void rscf_default_ctrl(rscf_ctrl* c) {
     c->jk_prec = RJK_PREC_FP64;
     c->switch_diis_err = 1e-5;
     c->print_level = 0;
+    c->switch_mode = RSCF_SWITCH_SHIFT;
 }
 
 namespace {
@@ -263,11 +284,14 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     if (ctrl.num_max_diis < 1 || ctrl.num_max_diis > 16 || ctrl.max_scf_cycle < 1)
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": num_max_diis must be 1..16 and max_scf_cycle >= 1");
     if (!ctx->b64) return rjk::fail(ctx, RJK_ERR_STATE, std::string(who) + ": B is not set");
+    if (ctrl.jk_prec != RJK_PREC_FP64 && ctrl.jk_prec != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec))
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64, FP32, BF16X2 or BF16X3");
     if (ctrl.jk_prec == RJK_PREC_FP32 && !ctx->b32)
         return rjk::fail(ctx, RJK_ERR_STATE,
                          std::string(who) + ": mixed precision needs the FP32 copy of B (RJK_STORE_FP32)");
-    if (ctrl.jk_prec != RJK_PREC_FP64 && ctrl.jk_prec != RJK_PREC_FP32)
-        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64 or FP32");
+    if (rjk::is_bf16(ctrl.jk_prec) && !ctx->bpl)
+        return rjk::fail(ctx, RJK_ERR_STATE,
+                         std::string(who) + ": BF16 mixed precision needs the BF16 planes of B (RJK_STORE_BF16)");
     if (guess_only && !d_guess) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": a guess density is required");
     if (d_guess && c_guess) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": give a guess density or orbitals");
     const double t_start = now();
@@ -292,9 +316,10 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     double* info_d = mem.get(ctx, 1, &rc);
     // per spin: Fock, extrapolated Fock, eigenvectors, densities (current, previous,
     // mixed), eigenvalues, occupations, exchange
-    double *F[2], *Fx[2], *A[2], *D[2], *Dp[2], *Dm[2], *W[2], *occd[2], *vk[2];
+    double *F[2], *Fx[2], *A[2], *D[2], *Dp[2], *Dm[2], *W[2], *occd[2], *vk[2], *Fl[2];
     for (int s = 0; s < nspin; ++s) {
         F[s] = mem.get(ctx, n2, &rc);
+        Fl[s] = mem.get(ctx, n2, &rc);      // low-precision Fock at the switch
         Fx[s] = mem.get(ctx, n2, &rc);
         A[s] = mem.get(ctx, n2, &rc);
         D[s] = mem.get(ctx, n2, &rc);
@@ -308,7 +333,10 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     const size_t ev_len = (size_t)nspin * n2;   // error vector: spins back to back (REST :3133)
     double* ering = mem.get(ctx, (size_t)nd * ev_len, &rc);
     double* fring = mem.get(ctx, (size_t)nd * ev_len, &rc);
+    double* dring = mem.get(ctx, (size_t)nd * ev_len, &rc);   // the densities of the stored Fock matrices
     if (rc) return rc;
+    if (ctrl.switch_mode != RSCF_SWITCH_RESTART && ctrl.switch_mode != RSCF_SWITCH_SHIFT)
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": switch_mode must be RSCF_SWITCH_RESTART or _SHIFT");
     SCF_HIP(hipMemcpyAsync(S, S_h, n2 * sizeof(double), hipMemcpyHostToDevice, st));
     SCF_HIP(hipMemcpyAsync(H, H_h, n2 * sizeof(double), hipMemcpyHostToDevice, st));
     for (int s = 0; s < nspin; ++s) {
@@ -507,6 +535,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     for (int s = 0; s < nspin; ++s) eps_prev[s] = eps[s];
     int num_iter = 1, n_fp64 = 0, start_diis = ctrl.start_diis_cycle;
     int nvec = 0, head = 0;   // DIIS ring: vectors [head, head+nvec) mod nd
+    bool shift_pending = false;   // first FP64 build after the switch: shift the stored vectors
     std::vector<double> Bmat((size_t)nd * nd, 0.0);
     bool converged = false;
     if (ctrl.print_level > 0) std::printf("rscf: guess energy %.10f\n", E);
@@ -527,10 +556,52 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             return RJK_OK;
         };
         if (num_iter < start_diis) {
+            shift_pending = false;
             SCF_RC(linear_mix());
         } else {
+            if (shift_pending) {   // Flow = F at the low precision, same density
+                double e_low = 0.0;
+                SCF_RC(fock(D, true, ctrl.jk_prec, &e_low));
+                for (int s = 0; s < nspin; ++s)
+                    SCF_HIP(hipMemcpyAsync(Fl[s], F[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+            }
             SCF_RC(fock(D, true, prec, &E));
             const double t0 = now();
+            if (shift_pending && nvec > 0) {
+                // δ_s = F_s(FP64) − F_s(low) at the current density.  The low-precision
+                // error of F is a smooth function of D, so every stored F_k(low) + δ is
+                // F_k(FP64) up to (low-precision error) × |D_k − D|, and its error vector
+                // becomes e_k + δ D_k S − S D_k δ.
+                for (int s = 0; s < nspin; ++s) {
+                    hipLaunchKernelGGL(k_delta_inplace, dim3(kRedBlocks), b256, 0, st, F[s], Fl[s], n2);
+                    SCF_HIP(hipGetLastError());
+                }
+                for (int k = 0; k < nvec; ++k) {
+                    const size_t slot_k = (size_t)((head + k) % nd) * ev_len;
+                    for (int s = 0; s < nspin; ++s) {
+                        hipLaunchKernelGGL(k_axpy_inplace, dim3(kRedBlocks), b256, 0, st, fring + slot_k + s * n2,
+                                           Fl[s], 1.0, n2);
+                        SCF_HIP(hipGetLastError());
+                        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
+                                               (rocblas_int)n, (rocblas_int)n, &one, Fl[s], (rocblas_int)n,
+                                               dring + slot_k + s * n2, (rocblas_int)n, &zero, T1, (rocblas_int)n));
+                        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
+                                               (rocblas_int)n, (rocblas_int)n, &one, T1, (rocblas_int)n, S,
+                                               (rocblas_int)n, &zero, T2, (rocblas_int)n));
+                        hipLaunchKernelGGL(k_commutator_acc, dim3(kRedBlocks), b256, 0, st, T2, n,
+                                           ering + slot_k + s * n2);
+                        SCF_HIP(hipGetLastError());
+                    }
+                }
+                for (int i = 0; i < nvec; ++i)
+                    for (int j = 0; j <= i; ++j) {
+                        double dot = 0.0;
+                        SCF_BLAS(rocblas_ddot(h, (rocblas_int)ev_len, ering + (size_t)((head + i) % nd) * ev_len, 1,
+                                              ering + (size_t)((head + j) % nd) * ev_len, 1, &dot));
+                        Bmat[(size_t)i * nd + j] = Bmat[(size_t)j * nd + i] = dot;
+                    }
+            }
+            shift_pending = false;
             // error vector e_s = F_s D_s S − S D_s F_s, spins back to back, into the ring slot
             if (nvec == nd) {   // drop the oldest
                 head = (head + 1) % nd;
@@ -553,6 +624,8 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                 SCF_RC(reduce_max(&m));
                 diis_max = std::max(diis_max, m);
                 SCF_HIP(hipMemcpyAsync(fring + (size_t)slot * ev_len + (size_t)s * n2, F[s], n2 * sizeof(double),
+                                       hipMemcpyDeviceToDevice, st));
+                SCF_HIP(hipMemcpyAsync(dring + (size_t)slot * ev_len + (size_t)s * n2, D[s], n2 * sizeof(double),
                                        hipMemcpyDeviceToDevice, st));
             }
             ++nvec;
@@ -587,12 +660,18 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             }
             t_diis += now() - t0;
             // precision switch: FP64 J/K from the next build on.  The stored Fock
-            // matrices and error vectors carry the FP32 J/K error (about the size of
-            // the switch threshold), so DIIS restarts with FP64 vectors only.
+            // matrices and error vectors carry the low-precision J/K error (about the
+            // size of the switch threshold): either DIIS restarts with FP64 vectors
+            // only, or the next iteration shifts them by F(FP64) − F(low) measured at
+            // its density (one extra low-precision build).
             if (prec != RJK_PREC_FP64 && diis_max < ctrl.switch_diis_err) {
                 prec = RJK_PREC_FP64;
-                nvec = 0;
-                head = 0;
+                if (ctrl.switch_mode == RSCF_SWITCH_SHIFT) {
+                    shift_pending = nvec > 0;
+                } else {
+                    nvec = 0;
+                    head = 0;
+                }
             }
         }
         prec_E = prec_used;
@@ -630,7 +709,8 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         }
         if (ctrl.print_level > 0)
             std::printf("rscf: iter %3d  E %.10f  dE %.3e  diis %.3e  dD %.3e  eev %.3e  prec %s\n", num_iter, E, dE,
-                        diis_max, dm_max, eev, prec_used == RJK_PREC_FP64 ? "fp64" : "fp32");
+                        diis_max, dm_max, eev,
+                        prec_used == RJK_PREC_FP64 ? "fp64" : (prec_used == RJK_PREC_FP32 ? "fp32" : "bf16"));
         E_prev = E;
         prec_prev = prec_E;
         for (int s = 0; s < nspin; ++s) eps_prev[s] = eps[s];

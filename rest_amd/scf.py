@@ -21,7 +21,10 @@ class Ctrl(C.Structure):
     _fields_ = [("num_max_diis", C.c_int), ("start_diis_cycle", C.c_int), ("mix_param", C.c_double),
                 ("max_scf_cycle", C.c_int), ("scf_acc_rho", C.c_double), ("scf_acc_eev", C.c_double),
                 ("scf_acc_etot", C.c_double), ("jk_prec", C.c_int), ("switch_diis_err", C.c_double),
-                ("print_level", C.c_int)]
+                ("print_level", C.c_int), ("switch_mode", C.c_int)]
+
+
+SWITCH_RESTART, SWITCH_SHIFT = 0, 1
 
 
 class Result(C.Structure):
@@ -208,7 +211,8 @@ def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctr
     H = T + V
     t1 = time.perf_counter()
     ri_ = jk.DeviceRI(mol.nbf, mol.naux)
-    store = jk.STORE_FP64 | (jk.STORE_FP32 if prec == jk.PREC_FP32 else 0)
+    store = jk.STORE_FP64 | (jk.STORE_FP32 if prec != jk.PREC_FP64 else 0) \
+        | (jk.STORE_BF16 if prec in (jk.PREC_BF16X2, jk.PREC_BF16X3) else 0)
     binfo = mol.build_rimatr(ri_, threshold, store)
     t2 = time.perf_counter()
     c_guess = None

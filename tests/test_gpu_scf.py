@@ -82,19 +82,47 @@ def test_water_cluster_fp64_vs_oracle(seed, n):
     np.testing.assert_allclose(out.dm, 2 * C[:, :nocc] @ C[:, :nocc].T, atol=1e-10)
 
 
-def test_mixed_precision_converges_to_fp64_energy():
-    say("(H2O)4 mixed precision (FP32 J/K -> FP64 at max|e| < 1e-5)")
+_E64 = {}
+
+
+@pytest.mark.parametrize("prec,mode", [(jk.PREC_FP32, rs.SWITCH_SHIFT), (jk.PREC_FP32, rs.SWITCH_RESTART),
+                                       (jk.PREC_BF16X3, rs.SWITCH_SHIFT), (jk.PREC_BF16X3, rs.SWITCH_RESTART)])
+def test_mixed_precision_converges_to_fp64_energy(prec, mode):
+    say(f"(H2O)4 mixed precision (prec {prec} J/K -> FP64 at max|e| < 1e-5), switch mode {mode}")
     m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
-    o64, _ = rs.run_molecule(m, ctrl=tight())
-    o32, _ = rs.run_molecule(m, prec=jk.PREC_FP32, ctrl=tight())
-    precs = [int(r["prec"]) for r in o32.log]
-    say(f"  fp64 {o64.energy:.12f} ({o64.iterations} it)  mixed {o32.energy:.12f} ({o32.iterations} it, "
-        f"{o32.fp64_iterations} fp64)  dE {o32.energy - o64.energy:.2e}")
-    assert o32.converged and abs(o32.energy - o64.energy) <= 1e-9
-    assert precs[0] == jk.PREC_FP32 and precs[-1] == jk.PREC_FP64
+    if "h2o4" not in _E64:
+        o64, _ = rs.run_molecule(m, ctrl=tight())
+        assert o64.converged
+        _E64["h2o4"] = (o64.energy, o64.iterations)
+    e64, it64 = _E64["h2o4"]
+    ctrl = tight()
+    ctrl.switch_mode = mode
+    om, _ = rs.run_molecule(m, prec=prec, ctrl=ctrl)
+    precs = [int(r["prec"]) for r in om.log]
+    say(f"  fp64 {e64:.12f} ({it64} it)  mixed {om.energy:.12f} ({om.iterations} it, "
+        f"{om.fp64_iterations} fp64)  dE {om.energy - e64:.2e}")
+    assert om.converged and abs(om.energy - e64) <= 1e-9
+    assert precs[0] == prec and precs[-1] == jk.PREC_FP64
     # once switched, the J/K stay FP64
     first64 = precs.index(jk.PREC_FP64)
     assert all(p == jk.PREC_FP64 for p in precs[first64:])
+
+
+def test_mixed_uhf_shift_matches_fp64():
+    say("H2O+ doublet UHF: BF16X3 -> FP64 with the shifted DIIS history vs plain FP64")
+    m = ri.Molecule.water_cluster(wl.water_cluster(3, 1))
+    S, T, V = m.int1e()
+    H = T + V
+    enuc = m.nuclear_repulsion()
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        o64 = rs.uhf(r, S, H, enuc, 5, 4, None, tight())
+        ctrl = tight()
+        ctrl.jk_prec = jk.PREC_BF16X3
+        om = rs.uhf(r, S, H, enuc, 5, 4, None, ctrl)
+    say(f"  fp64 {o64.energy:.12f} ({o64.iterations} it)  mixed {om.energy:.12f} ({om.iterations} it, "
+        f"{om.fp64_iterations} fp64)")
+    assert o64.converged and om.converged and abs(om.energy - o64.energy) <= 1e-9
 
 
 def test_scf_error_paths():
@@ -112,5 +140,11 @@ def test_scf_error_paths():
         with pytest.raises(jk.RJKError) as e:
             rs.rhf(r, S, T + V, 0.0, 10, ctrl=rs.default_ctrl(jk_prec=jk.PREC_FP32))
         assert e.value.code == 4          # no FP32 copy of B
+        with pytest.raises(jk.RJKError) as e:
+            rs.rhf(r, S, T + V, 0.0, 10, ctrl=rs.default_ctrl(jk_prec=jk.PREC_BF16X3))
+        assert e.value.code == 4          # no BF16 planes of B
+        with pytest.raises(jk.RJKError) as e:
+            rs.rhf(r, S, T + V, 0.0, 10, ctrl=rs.default_ctrl(switch_mode=5))
+        assert e.value.code == 1
         out = rs.rhf(r, S, T + V, m.nuclear_repulsion(), 10)
         assert out.converged
