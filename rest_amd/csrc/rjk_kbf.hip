@@ -20,19 +20,14 @@
 
 #include <algorithm>
 
+#include "rjk_bftile.hpp"
 #include "rjk_internal.hpp"
 
 namespace rjk {
 namespace {
 
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
-constexpr int kThreads = 256;   // 4 waves, 2 × 2
-constexpr int TKB = 32;         // k-block: one MFMA deep
-constexpr int RS = TKB + 8;     // LDS row stride in elements (80 bytes: rows stay 16-byte aligned)
 constexpr int kXcdB = 8;
 
 __device__ inline size_t tri_b(size_t j) { return j * (j + 1) / 2; }
@@ -58,17 +53,6 @@ __device__ inline void split_f(float x, __bf16* out) {
         r -= (float)h;
     }
 }
-
-// plane pair (first operand plane, second operand plane) of product p
-template <int S> __device__ constexpr int pa(int p);
-template <int S> __device__ constexpr int pb(int p);
-template <> __device__ constexpr int pa<2>(int p) { return p == 2 ? 1 : 0; }
-template <> __device__ constexpr int pb<2>(int p) { return p == 1 ? 1 : 0; }
-template <> __device__ constexpr int pa<3>(int p) { return p == 2 || p == 4 ? 1 : (p == 5 ? 2 : 0); }
-template <> __device__ constexpr int pb<3>(int p) { return p == 1 || p == 4 ? 1 : (p == 3 ? 2 : 0); }
-template <int S> constexpr int npairs() { return S * (S + 1) / 2; }
-
-__device__ inline f4 mfma(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 // ---- conversions ---------------------------------------------------------------
 // B planes of auxiliary columns [P0, P0 + Pc) from the packed FP64 B (upper triangle,
@@ -141,87 +125,46 @@ __global__ void k_bf_prep_c_idx(const double* __restrict__ vecs, int nbf, const 
     }
 }
 
-__device__ inline bf8 frag(const __bf16* row) { return *(const bf8*)row; }
+// Tile shapes (measured at nbf 1536, nk 384, 738 auxiliary columns per launch, against
+// 64c × 128μ / 32-deep and 128 × 128 / 32-deep with padded LDS rows, and hipBLASLt's
+// batched BF16 GEMM of the same shape at 883 TFLOP/s): half-transform 931 TFLOP/s,
+// SYRK 981 TFLOP/s of executed MFMA work.
+template <int S> using KhalfCfg = bft::Cfg<S, 64, 128, 2, 2, false, 64>;    // X = 64 c, Y = 128 μ
+template <int S> using KsyrkCfg = bft::Cfg<S, 128, 128, 2, 2, false, 32>;   // X = 128 ν, Y = 128 μ
 
 // ---- stage 1 -------------------------------------------------------------------
 // 1-D grid, XCD-aware: every tile of auxiliary column P runs on XCD P % 8 and the
 // column tiles of one μ tile are consecutive there, so they share B's rows in L2.
-// Work-group tile 128 μ × 64 c; wave (wm, wn) computes 64 μ × 32 c.
 template <int S>
-__global__ __launch_bounds__(kThreads) void k_bf_khalf(const __bf16* __restrict__ bpl, size_t naux, int nbf,
-                                                       int nbf_pad, size_t P0, int Pc, int ntm, int ntc,
-                                                       const __bf16* __restrict__ ct, int nk,
-                                                       __bf16* __restrict__ mpl) {
-    constexpr int NP = npairs<S>();
-    __shared__ __bf16 Bs[S][128][RS];   // [μ][ν]
-    __shared__ __bf16 Cs[S][64][RS];    // [c][ν]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+__global__ __launch_bounds__(KhalfCfg<S>::NT) void k_bf_khalf(const __bf16* __restrict__ bpl, size_t naux, int nbf,
+                                                              int nbf_pad, size_t P0, int Pc, int ntm, int ntc,
+                                                              const __bf16* __restrict__ ct, int nk,
+                                                              __bf16* __restrict__ mpl) {
+    using C = KhalfCfg<S>;
+    __shared__ __bf16 lds[C::LDS_ELEMS];
     const int w = blockIdx.x, xcd = w % kXcdB, kq = w / kXcdB, per = ntm * ntc;
     const int Pl = (kq / per) * kXcdB + xcd;
     if (Pl >= Pc) return;   // padding work-groups (uniform exit)
     const int rem = kq % per, mt = rem / ntc, ctile = rem % ntc;
-    const int mu0 = mt * 128, c0 = ctile * 64;
-    const size_t sq = (size_t)nbf_pad * nbf_pad, bstride = naux * sq, cplane = (size_t)nk * nbf_pad;
+    const int mu0 = mt * C::WGY, c0 = ctile * C::WGX;
+    const size_t sq = (size_t)nbf_pad * nbf_pad;
     const __bf16* __restrict__ bP = bpl + (P0 + Pl) * sq + (size_t)mu0 * nbf_pad;
     const __bf16* __restrict__ cT = ct + (size_t)c0 * nbf_pad;
-    // staging: row = tid / 4 (0..63), piece = 8 elements at (tid % 4) * 8;
-    // B rows row and row + 64, C' row row
-    const int row = tid >> 2, pc = (tid & 3) * 8;
-    u4 rb0[S], rb1[S], rc[S];
-    auto fetch = [&](int nu0) {
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const __bf16* b = bP + (size_t)k * bstride + nu0 + pc;
-            rb0[k] = *(const u4*)(b + (size_t)row * nbf_pad);
-            rb1[k] = *(const u4*)(b + (size_t)(row + 64) * nbf_pad);
-            rc[k] = *(const u4*)(cT + (size_t)k * cplane + (size_t)row * nbf_pad + nu0 + pc);
-        }
-    };
-    f4 acc[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    const int nkb = (nbf + TKB - 1) / TKB;
-    const int kk = 8 * (lane >> 4), l16 = lane & 15;
-    fetch(0);
-    for (int kb = 0; kb < nkb; ++kb) {
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            *(u4*)&Bs[k][row][pc] = rb0[k];
-            *(u4*)&Bs[k][row + 64][pc] = rb1[k];
-            *(u4*)&Cs[k][row][pc] = rc[k];
-        }
-        __syncthreads();
-        if (kb + 1 < nkb) fetch((kb + 1) * TKB);
-        bf8 fb[4][S], fc[2][S];
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) fb[i][k] = frag(&Bs[k][wm * 64 + i * 16 + l16][kk]);
-#pragma unroll
-            for (int j = 0; j < 2; ++j) fc[j][k] = frag(&Cs[k][wn * 32 + j * 16 + l16][kk]);
-        }
-        // D[c][μ] = Σ_ν C'ᵀ[c][ν] B[ν][μ]; products outermost so consecutive MFMAs
-        // write different accumulators
-#pragma unroll
-        for (int p = 0; p < NP; ++p)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = mfma(fc[j][pa<S>(p)], fb[i][pb<S>(p)], acc[i][j]);
-        __syncthreads();
-    }
+    bft::f4 acc[C::FX][C::FY];
+    // D[c][μ] = Σ_ν C'ᵀ[c][ν] B[μ][ν]  (B_P symmetric; padding rows and columns are zero)
+    bft::nt_tile<C>(lds, [&](int q) { return cT + q * C::TK; }, (size_t)nbf_pad, (size_t)nk * nbf_pad,
+                    [&](int q) { return bP + q * C::TK; }, (size_t)nbf_pad, naux * sq,
+                    [](int, int) { return bft::u4{0, 0, 0, 0}; }, (nbf + C::TK - 1) / C::TK, acc);
     // M' planes: lane holds c = cb + 4(l/16) + r for μ = mb + l%16
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wx = wave % C::WX, wy = wave / C::WX;
     const size_t mplane = (size_t)Pc * nbf_pad * nk;
     __bf16* __restrict__ mP = mpl + (size_t)Pl * nbf_pad * nk;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < C::FX; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int mu = mu0 + wm * 64 + i * 16 + l16;
-            const int c = c0 + wn * 32 + j * 16 + 4 * (lane >> 4);
+        for (int j = 0; j < C::FY; ++j) {
+            const int c = c0 + (wx * C::FX + i) * 16 + 4 * (lane >> 4);
+            const int mu = mu0 + (wy * C::FY + j) * 16 + (lane & 15);
             __bf16 h[4][S];
 #pragma unroll
             for (int r = 0; r < 4; ++r) split_f<S>(acc[i][j][r], h[r]);
@@ -233,18 +176,16 @@ __global__ __launch_bounds__(kThreads) void k_bf_khalf(const __bf16* __restrict_
 
 // ---- stage 2 -------------------------------------------------------------------
 // 1-D grid of nsplit × nspin × ntile2 work-groups (nsplit a multiple of 8; split s
-// on XCD s % 8).  Work-group tile 128 ν × 128 μ (μ tile <= ν tile); wave (wm, wn)
-// computes 64 ν × 64 μ.
+// on XCD s % 8).  Upper tile pairs (μ tile <= ν tile) of 128 × 128.
 template <int S>
-__global__ __launch_bounds__(kThreads) void k_bf_ksyrk(const __bf16* __restrict__ mpl, int Pc, int nbf_pad, int nk,
-                                                       int ntile2, int nspin, int off0, int cnt0, int off1, int cnt1,
-                                                       const unsigned short* __restrict__ smask, int p_per_split,
-                                                       int beta, double* __restrict__ kpart, size_t slot) {
-    constexpr int NP = npairs<S>();
-    __shared__ __bf16 As[S][128][RS];   // [ν][c], signs applied
-    __shared__ __bf16 Bs[S][128][RS];   // [μ][c]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+__global__ __launch_bounds__(KsyrkCfg<S>::NT) void k_bf_ksyrk(const __bf16* __restrict__ mpl, int Pc, int nbf_pad,
+                                                              int nk, int ntile2, int nspin, int off0, int cnt0,
+                                                              int off1, int cnt1,
+                                                              const unsigned short* __restrict__ smask,
+                                                              int p_per_split, int beta, double* __restrict__ kpart,
+                                                              size_t slot) {
+    using C = KsyrkCfg<S>;
+    __shared__ __bf16 lds[C::LDS_ELEMS];
     const int w = blockIdx.x, xcd = w % kXcdB, kq = w / kXcdB;
     const size_t t = (size_t)(kq % ntile2);
     const int rest = kq / ntile2, spin = rest % nspin, split = (rest / nspin) * kXcdB + xcd;
@@ -252,70 +193,32 @@ __global__ __launch_bounds__(kThreads) void k_bf_ksyrk(const __bf16* __restrict_
     while (tri_b(tn) > t) --tn;
     while (tri_b(tn + 1) <= t) ++tn;
     const size_t tm = t - tri_b(tn);
-    const int mu0 = (int)tm * 128, nu0 = (int)tn * 128;
+    const int mu0 = (int)tm * C::WGY, nu0 = (int)tn * C::WGX;
     const int cb = spin == 0 ? off0 : off1, cnt = spin == 0 ? cnt0 : cnt1;
-    const int ncb = (cnt + TKB - 1) / TKB;
+    const int ncb = (cnt + C::TK - 1) / C::TK;
     const int Pb = split * p_per_split, Pe = min(Pc, Pb + p_per_split);
     const int nq = (Pe > Pb ? Pe - Pb : 0) * ncb;
-    const size_t mplane = (size_t)Pc * nbf_pad * nk;
-    const int row = tid >> 2, pc = (tid & 3) * 8;
-    u4 ra0[S], ra1[S], rb0[S], rb1[S], rm;
-    auto fetch = [&](int q) {
-        const int P = Pb + q / ncb, c = cb + (q % ncb) * TKB + pc;
-        const __bf16* base = mpl + (size_t)P * nbf_pad * nk + c;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const __bf16* bk = base + (size_t)k * mplane;
-            ra0[k] = *(const u4*)(bk + (size_t)(nu0 + row) * nk);
-            ra1[k] = *(const u4*)(bk + (size_t)(nu0 + row + 64) * nk);
-            rb0[k] = *(const u4*)(bk + (size_t)(mu0 + row) * nk);
-            rb1[k] = *(const u4*)(bk + (size_t)(mu0 + row + 64) * nk);
-        }
-        const unsigned short* sm = smask + c;
-        rm = u4{(unsigned)sm[0] | ((unsigned)sm[1] << 16), (unsigned)sm[2] | ((unsigned)sm[3] << 16),
-                (unsigned)sm[4] | ((unsigned)sm[5] << 16), (unsigned)sm[6] | ((unsigned)sm[7] << 16)};
-    };
-    f4 acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    const int kk = 8 * (lane >> 4), l16 = lane & 15;
-    if (nq > 0) fetch(0);
-    for (int q = 0; q < nq; ++q) {
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            *(u4*)&As[k][row][pc] = ra0[k] ^ rm;   // sign_c: flip the sign bits exactly
-            *(u4*)&As[k][row + 64][pc] = ra1[k] ^ rm;
-            *(u4*)&Bs[k][row][pc] = rb0[k];
-            *(u4*)&Bs[k][row + 64][pc] = rb1[k];
-        }
-        __syncthreads();
-        if (q + 1 < nq) fetch(q + 1);
-        bf8 fa[4][S], fb[4][S];
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                fa[i][k] = frag(&As[k][wn * 64 + i * 16 + l16][kk]);
-                fb[i][k] = frag(&Bs[k][wm * 64 + i * 16 + l16][kk]);
-            }
-        // D[ν][μ] = Σ_c sign_c M'[ν][c] M'[μ][c]
-#pragma unroll
-        for (int p = 0; p < NP; ++p)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = mfma(fa[j][pa<S>(p)], fb[i][pb<S>(p)], acc[i][j]);
-        __syncthreads();
-    }
+    const size_t mplane = (size_t)Pc * nbf_pad * nk, mP = (size_t)nbf_pad * nk;
+    bft::f4 acc[C::FX][C::FY];
+    // D[ν][μ] = Σ_c sign_c M'[ν][c] M'[μ][c]
+    bft::nt_tile<C>(
+        lds, [&](int q) { return mpl + (Pb + q / ncb) * mP + (size_t)nu0 * nk + cb + (q % ncb) * C::TK; }, (size_t)nk,
+        mplane, [&](int q) { return mpl + (Pb + q / ncb) * mP + (size_t)mu0 * nk + cb + (q % ncb) * C::TK; },
+        (size_t)nk, mplane,
+        [&](int q, int pc) {
+            const unsigned short* sm = smask + cb + (q % ncb) * C::TK + pc;
+            return bft::u4{(unsigned)sm[0] | ((unsigned)sm[1] << 16), (unsigned)sm[2] | ((unsigned)sm[3] << 16),
+                           (unsigned)sm[4] | ((unsigned)sm[5] << 16), (unsigned)sm[6] | ((unsigned)sm[7] << 16)};
+        },
+        nq, acc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wx = wave % C::WX, wy = wave / C::WX;
     double* __restrict__ kp = kpart + ((size_t)split * nspin + spin) * slot;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < C::FX; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int mu = mu0 + wm * 64 + i * 16 + l16;
-            const int nb = nu0 + wn * 64 + j * 16 + 4 * (lane >> 4);
+        for (int j = 0; j < C::FY; ++j) {
+            const int nb = nu0 + (wx * C::FX + i) * 16 + 4 * (lane >> 4);
+            const int mu = mu0 + (wy * C::FY + j) * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 double* dst = kp + (size_t)(nb + r) * nbf_pad + mu;
@@ -382,11 +285,11 @@ hipError_t launch_bf_khalf(hipStream_t st, int prec, const void* bpl, size_t nau
     const int ntm = (int)(kp.nbf_pad / 128), ntc = (int)(kp.nk / 64);
     const size_t nwg = ((Pc + kXcdB - 1) / kXcdB * kXcdB) * (size_t)ntm * ntc;
     if (bf_planes(prec) == 3)
-        hipLaunchKernelGGL(k_bf_khalf<3>, dim3((unsigned)nwg), dim3(kThreads), 0, st, (const __bf16*)bpl, naux,
+        hipLaunchKernelGGL(k_bf_khalf<3>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)bpl, naux,
                            (int)nbf, (int)kp.nbf_pad, P0, (int)Pc, ntm, ntc, (const __bf16*)ct, (int)kp.nk,
                            (__bf16*)mpl);
     else
-        hipLaunchKernelGGL(k_bf_khalf<2>, dim3((unsigned)nwg), dim3(kThreads), 0, st, (const __bf16*)bpl, naux,
+        hipLaunchKernelGGL(k_bf_khalf<2>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)bpl, naux,
                            (int)nbf, (int)kp.nbf_pad, P0, (int)Pc, ntm, ntc, (const __bf16*)ct, (int)kp.nk,
                            (__bf16*)mpl);
     return hipGetLastError();
@@ -398,12 +301,12 @@ hipError_t launch_bf_ksyrk(hipStream_t st, int prec, const void* mpl, size_t Pc,
     const int pps = (int)((Pc + kp.nsplit - 1) / kp.nsplit);
     const size_t nwg = kp.nsplit * (size_t)nspin * nt2;
     if (bf_planes(prec) == 3)
-        hipLaunchKernelGGL(k_bf_ksyrk<3>, dim3((unsigned)nwg), dim3(kThreads), 0, st, (const __bf16*)mpl, (int)Pc,
+        hipLaunchKernelGGL(k_bf_ksyrk<3>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)mpl, (int)Pc,
                            (int)kp.nbf_pad, (int)kp.nk, (int)nt2, nspin, (int)kp.col_off[0], (int)kp.col_cnt[0],
                            (int)kp.col_off[1], (int)kp.col_cnt[1], (const unsigned short*)smask, pps, beta, kpart,
                            kp.nbf_pad * kp.nbf_pad);
     else
-        hipLaunchKernelGGL(k_bf_ksyrk<2>, dim3((unsigned)nwg), dim3(kThreads), 0, st, (const __bf16*)mpl, (int)Pc,
+        hipLaunchKernelGGL(k_bf_ksyrk<2>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)mpl, (int)Pc,
                            (int)kp.nbf_pad, (int)kp.nk, (int)nt2, nspin, (int)kp.col_off[0], (int)kp.col_cnt[0],
                            (int)kp.col_off[1], (int)kp.col_cnt[1], (const unsigned short*)smask, pps, beta, kpart,
                            kp.nbf_pad * kp.nbf_pad);
