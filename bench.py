@@ -183,9 +183,12 @@ def main_scf(args):
         return out, {"guess": "sad", "e_guess": e0}
 
     def mixed():
-        g = lg.learned_guess(net, mol, r, S, H, mixed_prec)
-        out = rs.rhf(r, S, H, enuc, mol.nelectron, None, rs.default_ctrl(jk_prec=mixed_prec), c_guess=g.c1)
-        return out, {"guess": g.kind, "e_learned": g.e_learned, "e_sad": g.e_sad, "guess_seconds": g.seconds}
+        ctrl = rs.default_ctrl(jk_prec=mixed_prec)
+        if args.mixed_prec.startswith("bf16"):
+            ctrl.jk_prec_start = jk.PREC_BF16X2        # BF16X2 until max|e| < 1e-3
+        g = lg.learned_scf(net, mol, r, S, H, ctrl)
+        return g.outcome, {"guess": g.kind, "e_learned": g.e_learned, "e_sad": g.e_sad, "seconds": g.seconds,
+                           "precisions": [int(x["prec"]) for x in g.outcome.log]}
 
     for _ in range(args.warmup):
         plain()

@@ -42,7 +42,12 @@ typedef struct rscf_ctrl {
     int jk_prec;             /* RJK_PREC_FP64 (plain) or FP32 / BF16X3 / BF16X2 (mixed) */
     double switch_diis_err;  /* mixed mode: switch to FP64 below this max|e| (1e-5) */
     int print_level;         /* 0 silent, 1 one line per iteration on stdout */
-    int switch_mode;         /* mixed mode, at the switch: RSCF_SWITCH_SHIFT (default) or _RESTART */
+    int switch_mode;         /* mixed mode, at the switch: RSCF_SWITCH_RESTART (default) or _SHIFT */
+    int jk_prec_start;       /* builds run at this precision until max|e| < start_switch_err, then
+                              * at jk_prec (no DIIS restart); -1 (default): jk_prec throughout */
+    double start_switch_err; /* 1e-3 */
+    double guess_energy_max; /* return before the first iteration when the guess Fock build gives
+                              * E(guess) >= this (result.guess_rejected = 1); HUGE_VAL (default): never */
 } rscf_ctrl;
 
 /* What DIIS keeps at the precision switch.  RESTART: the stored low-precision
@@ -63,6 +68,8 @@ typedef struct rscf_result {
     int fp64_iterations;     /* of which with FP64 J/K                     */
     double wall_seconds;     /* whole driver, guess Fock build included    */
     double jk_seconds, diag_seconds, diis_seconds;
+    double guess_energy;     /* energy of the guess Fock build (the given density or orbitals) */
+    int guess_rejected;      /* 1: guess_energy >= ctrl.guess_energy_max and no iteration ran */
 } rscf_result;
 
 void rscf_default_ctrl(rscf_ctrl* ctrl);

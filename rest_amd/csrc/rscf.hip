@@ -247,7 +247,10 @@ extern "C" void rscf_default_ctrl(rscf_ctrl* c) {
     c->jk_prec = RJK_PREC_FP64;
     c->switch_diis_err = 1e-5;
     c->print_level = 0;
-    c->switch_mode = RSCF_SWITCH_SHIFT;
+    c->switch_mode = RSCF_SWITCH_RESTART;
+    c->jk_prec_start = -1;
+    c->start_switch_err = 1e-3;
+    c->guess_energy_max = HUGE_VAL;
 }
 
 namespace {
@@ -286,12 +289,16 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     if (!ctx->b64) return rjk::fail(ctx, RJK_ERR_STATE, std::string(who) + ": B is not set");
     if (ctrl.jk_prec != RJK_PREC_FP64 && ctrl.jk_prec != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec))
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64, FP32, BF16X2 or BF16X3");
-    if (ctrl.jk_prec == RJK_PREC_FP32 && !ctx->b32)
-        return rjk::fail(ctx, RJK_ERR_STATE,
-                         std::string(who) + ": mixed precision needs the FP32 copy of B (RJK_STORE_FP32)");
-    if (rjk::is_bf16(ctrl.jk_prec) && !ctx->bpl)
-        return rjk::fail(ctx, RJK_ERR_STATE,
-                         std::string(who) + ": BF16 mixed precision needs the BF16 planes of B (RJK_STORE_BF16)");
+    if (ctrl.jk_prec_start != -1 && ctrl.jk_prec_start != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec_start))
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec_start must be -1, FP32, BF16X2 or BF16X3");
+    for (int pr : {ctrl.jk_prec, ctrl.jk_prec_start}) {
+        if (pr == RJK_PREC_FP32 && !ctx->b32)
+            return rjk::fail(ctx, RJK_ERR_STATE,
+                             std::string(who) + ": mixed precision needs the FP32 copy of B (RJK_STORE_FP32)");
+        if (rjk::is_bf16(pr) && !ctx->bpl)
+            return rjk::fail(ctx, RJK_ERR_STATE,
+                             std::string(who) + ": BF16 mixed precision needs the BF16 planes of B (RJK_STORE_BF16)");
+    }
     if (guess_only && !d_guess) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": a guess density is required");
     if (d_guess && c_guess) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": give a guess density or orbitals");
     const double t_start = now();
@@ -493,9 +500,23 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         }
         guess_from_orbitals = true;
     }
-    int prec = ctrl.jk_prec;
+    int prec = ctrl.jk_prec_start >= 0 ? ctrl.jk_prec_start : ctrl.jk_prec;
+    int prec_low = prec;   // the precision the FP64 switch leaves
     double E = 0.0;
     SCF_RC(fock(Dp, guess_from_orbitals, prec, &E));
+    const double E_guess = E;
+    if (!energy_only && !guess_only && E >= ctrl.guess_energy_max) {   // guess not good enough: stop here
+        SCF_HIP(hipStreamSynchronize(st));
+        if (result) {
+            *result = rscf_result{};
+            result->energy = E;
+            result->guess_energy = E;
+            result->guess_rejected = 1;
+            result->wall_seconds = now() - t_start;
+            result->jk_seconds = t_jk;
+        }
+        return RJK_OK;
+    }
     if (energy_only) {
         SCF_HIP(hipStreamSynchronize(st));
         if (result) {
@@ -561,7 +582,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         } else {
             if (shift_pending) {   // Flow = F at the low precision, same density
                 double e_low = 0.0;
-                SCF_RC(fock(D, true, ctrl.jk_prec, &e_low));
+                SCF_RC(fock(D, true, prec_low, &e_low));
                 for (int s = 0; s < nspin; ++s)
                     SCF_HIP(hipMemcpyAsync(Fl[s], F[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
             }
@@ -664,7 +685,13 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             // size of the switch threshold): either DIIS restarts with FP64 vectors
             // only, or the next iteration shifts them by F(FP64) − F(low) measured at
             // its density (one extra low-precision build).
-            if (prec != RJK_PREC_FP64 && diis_max < ctrl.switch_diis_err) {
+            // The start precision gives way to jk_prec below start_switch_err without a
+            // DIIS restart (its error is far below that threshold).
+            const bool to64 = prec != RJK_PREC_FP64 &&
+                              (diis_max < ctrl.switch_diis_err ||
+                               (ctrl.jk_prec == RJK_PREC_FP64 && diis_max < ctrl.start_switch_err));
+            if (to64) {
+                prec_low = prec;
                 prec = RJK_PREC_FP64;
                 if (ctrl.switch_mode == RSCF_SWITCH_SHIFT) {
                     shift_pending = nvec > 0;
@@ -672,6 +699,8 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                     nvec = 0;
                     head = 0;
                 }
+            } else if (ctrl.jk_prec_start >= 0 && prec == ctrl.jk_prec_start && diis_max < ctrl.start_switch_err) {
+                prec = ctrl.jk_prec;   // one step up only: FP64 never gives way again
             }
         }
         prec_E = prec_used;
@@ -727,6 +756,8 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         result->jk_seconds = t_jk;
         result->diag_seconds = t_diag;
         result->diis_seconds = t_diis;
+        result->guess_energy = E_guess;
+        result->guess_rejected = 0;
     }
     for (int s = 0; s < nspin; ++s) {
         if (mo_coeff) SCF_HIP(hipMemcpy(mo_coeff + (size_t)s * n2, A[s], n2 * sizeof(double), hipMemcpyDeviceToHost));

@@ -630,6 +630,57 @@ def learned_guess(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, prec: in
     return LearnedGuess(c1_l if ok else c1_s, "learned" if ok else "sad", e_l, e_s, ne, secs)
 
 
+@dataclass
+class LearnedScf:
+    outcome: "rs.ScfOutcome"    # the SCF that was kept
+    kind: str                   # "learned" or "sad"
+    e_learned: float            # E(D) of the purified prediction (the SCF's guess Fock build)
+    e_sad: float                # E(D1) of SAD's first orbital density
+    electrons: float            # 2 tr(PS) of the purified prediction
+    seconds: dict = field(default_factory=dict)
+
+
+def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
+                mcweeny_iterations: int = 6) -> LearnedScf:
+    """RHF from the learned guess, with the acceptance rule of learned_guess folded into the SCF.
+
+    SAD's E(D1) is computed first (REST's guess step from the SAD density plus the energy of the
+    new orbitals' density).  The SCF then starts from the purified prediction: its guess Fock
+    build is F(D_pred), whose energy decides: below E(D1_SAD) the SCF continues from there
+    (REST's guess-step sequence, without a separate guess-step build); otherwise it stops at once
+    and the SCF runs from SAD's first orbitals instead.  Both energies are evaluated at the SCF's
+    starting J/K precision."""
+    ctrl = ctrl or rs.default_ctrl()
+    prec0 = ctrl.jk_prec_start if ctrl.jk_prec_start >= 0 else ctrl.jk_prec
+    t0 = time.perf_counter()
+    da, db = gs.sad_density(mol)
+    d_sad = da + db
+    t1 = time.perf_counter()
+    d_pred = predict_density(net, mol, spherical_average(mol, d_sad))
+    d_pred *= mol.nelectron / float(np.sum(d_pred * S))
+    t2 = time.perf_counter()
+    d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
+    t3 = time.perf_counter()
+    enuc = mol.nuclear_repulsion()
+    _, _, c1_s, _, e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True)
+    t4 = time.perf_counter()
+    ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
+    out, e_l = None, float("inf")
+    if ok:
+        c2 = rs.Ctrl.from_buffer_copy(ctrl)
+        c2.guess_energy_max = e_s
+        out = rs.rhf(r, S, H, enuc, mol.nelectron, d_pur, c2)
+        e_l = out.guess_energy
+        if out.guess_rejected or not np.isfinite(e_l):
+            out = None
+    kind = "learned" if out is not None else "sad"
+    if out is None:
+        out = rs.rhf(r, S, H, enuc, mol.nelectron, None, ctrl, c_guess=c1_s)
+    t5 = time.perf_counter()
+    secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "scf": t5 - t4}
+    return LearnedScf(out, kind, e_l, e_s, ne, secs)
+
+
 # ---------------------------------------------------------------------------
 # training data from this SCF
 # ---------------------------------------------------------------------------

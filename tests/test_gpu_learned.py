@@ -136,3 +136,36 @@ def test_learned_guess_falls_back_to_sad_for_a_useless_network():
         f"SCF {out.energy:.10f} vs SAD-started {ref.energy:.10f}")
     assert (g.kind == "sad") == (not (g.e_learned < g.e_sad and abs(g.electrons - m.nelectron) < 0.5))
     assert out.converged and abs(out.energy - ref.energy) < 1e-9
+
+
+@pytest.mark.parametrize("use_model", [True, False])
+def test_learned_scf_matches_fp64_and_rejects_by_energy(use_model):
+    """learned_scf: SAD's E(D1) first, then the SCF from the purified prediction, kept only when its
+    guess Fock energy is below E(D1_SAD); BF16X2 start, BF16X3, then FP64.  The packaged model
+    (kept or not) and an untrained network (rejected or kept) both end at the FP64 SAD-started energy."""
+    from pathlib import Path
+    from rest_amd import guess as gs
+    from rest_amd import jk, ri
+    from rest_amd import scf as rs
+    from rest_amd import workload as wl
+    say(f"(H2O)8 learned_scf, {'packaged model' if use_model else 'untrained network'}")
+    m = ri.Molecule.water_cluster(wl.water_cluster(5, 8))
+    S, T_, V = m.int1e()
+    H = T_ + V
+    model = Path(lg.__file__).resolve().parent / "models" / "learned_guess_h2o.npz"
+    net = lg.Net.load(model) if use_model else lg.Net(seed=3)
+    tight = dict(scf_acc_etot=1e-11, scf_acc_rho=1e-9, scf_acc_eev=1e-8, max_scf_cycle=200)
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        ref = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, gs.sad_guess(m, r, S, H).d1,
+                     rs.default_ctrl(**tight))
+        ctrl = rs.default_ctrl(jk_prec=jk.PREC_BF16X3, jk_prec_start=jk.PREC_BF16X2, **tight)
+        g = lg.learned_scf(net, m, r, S, H, ctrl)
+    precs = [int(x["prec"]) for x in g.outcome.log]
+    say(f"  kind {g.kind}: E_learned {g.e_learned:.6f} E_sad {g.e_sad:.6f}; SCF {g.outcome.energy:.10f} "
+        f"({g.outcome.iterations} it, precisions {precs}) vs FP64 SAD-started {ref.energy:.10f}")
+    assert (g.kind == "learned") == (g.e_learned < g.e_sad)
+    assert g.outcome.converged and abs(g.outcome.energy - ref.energy) <= 1e-9
+    assert precs[-1] == jk.PREC_FP64
+    if use_model:
+        assert g.kind == "learned"

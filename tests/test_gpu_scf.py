@@ -108,6 +108,38 @@ def test_mixed_precision_converges_to_fp64_energy(prec, mode):
     assert all(p == jk.PREC_FP64 for p in precs[first64:])
 
 
+def test_precision_ladder_and_guess_rejection():
+    say("(H2O)4: BF16X2 start -> BF16X3 below 1e-3 -> FP64 below 1e-5; guess rejection by energy")
+    m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
+    S, T, V = m.int1e()
+    H = T + V
+    enuc = m.nuclear_repulsion()
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        o64 = rs.rhf(r, S, H, enuc, m.nelectron, None, tight())
+        ctrl = tight()
+        ctrl.jk_prec, ctrl.jk_prec_start = jk.PREC_BF16X3, jk.PREC_BF16X2
+        om = rs.rhf(r, S, H, enuc, m.nelectron, None, ctrl)
+        precs = [int(x["prec"]) for x in om.log]
+        diis = [x["diis_max"] for x in om.log]
+        say(f"  precisions {precs}; E {om.energy:.12f} vs {o64.energy:.12f}")
+        assert om.converged and abs(om.energy - o64.energy) <= 1e-9
+        assert precs[0] == jk.PREC_BF16X2 and precs[-1] == jk.PREC_FP64 and jk.PREC_BF16X3 in precs
+        # each precision change follows the DIIS error of the iteration before it
+        for i in range(1, len(precs)):
+            if precs[i] != precs[i - 1]:
+                assert diis[i - 1] < (1e-3 if precs[i] == jk.PREC_BF16X3 else 1e-5)
+        # guess rejection: the guess Fock energy is the core-Hamiltonian density's FP64 energy
+        c = tight()
+        c.guess_energy_max = -1e30
+        rej = rs.rhf(r, S, H, enuc, m.nelectron, None, c)
+        assert rej.guess_rejected and rej.iterations == 0 and not rej.converged
+        assert rej.guess_energy == pytest.approx(o64.guess_energy, abs=1e-9)
+        c.guess_energy_max = rej.guess_energy + 1e-6
+        kept = rs.rhf(r, S, H, enuc, m.nelectron, None, c)
+        assert not kept.guess_rejected and kept.converged and abs(kept.energy - o64.energy) <= 1e-9
+
+
 def test_mixed_uhf_shift_matches_fp64():
     say("H2O+ doublet UHF: BF16X3 -> FP64 with the shifted DIIS history vs plain FP64")
     m = ri.Molecule.water_cluster(wl.water_cluster(3, 1))
