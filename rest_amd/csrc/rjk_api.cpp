@@ -113,6 +113,7 @@ extern "C" int rjk_destroy(rjk_ctx* ctx) {
     if (ctx->b64) (void)hipFree(ctx->b64);
     if (ctx->b32) (void)hipFree(ctx->b32);
     if (ctx->bpl) (void)hipFree(ctx->bpl);
+    if (ctx->scf_x) (void)hipFree(ctx->scf_x);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->solver) rocblas_destroy_handle((rocblas_handle)ctx->solver);
     for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }

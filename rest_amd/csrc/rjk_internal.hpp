@@ -48,6 +48,8 @@ struct rjk_ctx {
     void* ws = nullptr;       // grow-only workspace
     size_t ws_size = 0;
     void* solver = nullptr;   // rocblas_handle, created on first density-only K
+    double* scf_x = nullptr;          // SCF driver: orthogonaliser S^-1/2 of the last S seen
+    std::vector<double> scf_s;        // that S (host copy, compared before reuse)
     std::string err;
     // phase timing
     bool prof = false;

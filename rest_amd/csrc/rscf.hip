@@ -383,9 +383,18 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     // F' = X F X (two GEMMs), F' C' = C' ε (rocSOLVER dsyevd), C = X C' (one GEMM).
     // Same eigenpairs as REST's generalised solver (dspgvx, src/scf_io/mod.rs:1851),
     // without re-factorising S in every iteration.
-    double* X = mem.get(ctx, n2, &rc);
-    if (rc) return rc;
-    {
+    // X cached on the context for the same S (the guess step and the SCF of one molecule share it)
+    double* X = nullptr;
+    if (ctx->scf_x && ctx->scf_s.size() == n2 && std::memcmp(ctx->scf_s.data(), S_h, n2 * sizeof(double)) == 0) {
+        X = ctx->scf_x;
+    } else {
+        if (ctx->scf_x) (void)hipFree(ctx->scf_x);
+        ctx->scf_x = nullptr;
+        ctx->scf_s.clear();
+        SCF_HIP(hipMalloc(&X, n2 * sizeof(double)));
+        ctx->scf_x = X;
+    }
+    if (ctx->scf_s.empty()) {
         SCF_HIP(hipMemcpyAsync(Bw, S, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
         SCF_BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, Bw, (rocblas_int)n,
                                   W[0], Ework, (rocblas_int*)info_d));
@@ -405,7 +414,10 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                                (rocblas_int)n, &one, T1, (rocblas_int)n, Bw, (rocblas_int)n, &zero, X,
                                (rocblas_int)n));
         SCF_HIP(hipStreamSynchronize(st));   // sv is a host temporary
+        ctx->scf_s.assign(S_h, S_h + n2);
     }
+    // F C = S C ε for spin s (FP64 throughout: eigenvectors from an FP32 solver at nbf ~1500 are
+    // orthonormal only to ~1e-5, which holds the DIIS error above the precision switch).
     auto diagonalise = [&](const double* Fm, int s) -> int {
         const double t0 = now();
         SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,

@@ -417,16 +417,18 @@ def read_chkfile(path, nbf: int, nstate: int, spin_channel: int = 1):
     eps = r.read("scf/mo_energy").ravel()
     cs, es = [], []
     for s in range(spin_channel):
-        blk = c[s * nstate * nbf:(s + 1) * nstate * nbf].reshape(nstate, nbf)   # [nstate][nbf]: Cᵀ row-major
-        cs.append(blk.T.copy())
+        # REST reads the block as a column-major [num_state, num_basis] matrix and transposes it
+        # (src/initial_guess/mod.rs:105-108): the bytes are C[ao][state] row-major, PySCF's layout too
+        cs.append(c[s * nstate * nbf:(s + 1) * nstate * nbf].reshape(nbf, nstate).copy())
         es.append(eps[s * nstate:(s + 1) * nstate].copy())
     return e, cs, es
 
 
 def write_chkfile(path, e_tot: float, mo_coeff: list, mo_energy: list):
-    """REST's save_chkfile layout: scf/e_tot, scf/mo_coeff (each spin's Cᵀ, row-major), scf/mo_energy."""
+    """REST's save_chkfile layout (src/post_scf_analysis/mod.rs:77-81): scf/e_tot, scf/mo_coeff (each
+    spin's C[ao][state] row-major: the data of the transposed column-major matrix), scf/mo_energy."""
     w = H5Writer()
     w.put("scf/e_tot", np.float64(e_tot))
-    w.put("scf/mo_coeff", np.concatenate([np.asarray(c, dtype=np.float64).T.ravel() for c in mo_coeff]))
+    w.put("scf/mo_coeff", np.concatenate([np.asarray(c, dtype=np.float64).ravel() for c in mo_coeff]))
     w.put("scf/mo_energy", np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in mo_energy]))
     w.save(path)

@@ -50,6 +50,10 @@ def test_pyscf_orbitals_in_this_builds_basis():
     r = h5.H5Reader(GOLD / "pen_pyscf_restart.chk")
     C = r.read("scf/mo_coeff")             # PySCF: [nao][nmo]
     occ = r.read("scf/mo_occ")
+    # the restart reader hands back the same [ao][state] orbitals (REST's reading of the raw data)
+    _, cs, es = h5.read_chkfile(GOLD / "pen_pyscf_restart.chk", m.nbf, C.shape[1], 1)
+    np.testing.assert_array_equal(cs[0], C)
+    np.testing.assert_array_equal(es[0], r.read("scf/mo_energy"))
     S, T, V = m.int1e()
     # PySCF converts Angstrom with 0.52917721092 bohr, REST with 0.5291772083 (src/constants/mod.rs:85):
     # the geometries differ by ~1e-9 relative, which the tolerances below allow for
