@@ -4,12 +4,16 @@
 
 --mode scf (default): the headline metric, SCF time to 1e-8 Eh for
   (H2O)64 def2-SVP / def2-SVP-JKFIT (water cluster seed 2): plain FP64 from
-  REST's SAD guess vs mixed precision (FP32 J/K until the DIIS error is below
-  1e-5, then FP64) from the learned guess, on the same GPU and the same
-  resident B.  One step = one plain-FP64 SCF plus one mixed+learned SCF;
-  `value` is their time ratio (plain / mixed+learned).  Integrals, the B
-  build and the atomic SAD densities are outside the timed region and
-  reported separately.
+  REST's SAD guess vs mixed precision from the learned guess, on the same GPU
+  and the same resident B.  Mixed: K on two BF16 planes of B (J on its FP32
+  copy) until the DIIS error max|FDS-SDF| is below 1e-3, then three planes
+  until 1e-5, then FP64; the learned density is kept only if its energy is
+  below that of SAD's first orbital density (both at the start precision).
+  One step = one plain-FP64 SCF plus one mixed+learned SCF, each including
+  its guess; `value` is their time ratio (plain / mixed+learned).
+  Integrals, the B build and the atomic SAD densities are outside the timed
+  region and reported separately.  --mixed-prec fp32 runs the mixed SCF on
+  the FP32 copy of B instead.
 --mode jk: one J + one K build (one SCF iteration's worth of the hot path)
   on synthetic B (seeded hash generator) at the (H2O)64 or (H2O)16 shape,
   D = C Cᵀ from the seeded orthonormal Gaussian C, occupied-route K.
@@ -241,8 +245,8 @@ def main_scf(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (plain); " + ("f32" if args.mixed_prec == "fp32" else "split-bf16 K (3 planes) + f32 J")
-                 + "->f64 J/K, f64 elsewhere (mixed)",
+        "dtype": "f64 (plain); " + ("f32" if args.mixed_prec == "fp32" else "split-bf16 K (2, then 3 planes) + f32 J")
+                 + " -> f64 J/K, f64 elsewhere (mixed)",
         "data": "synthetic (seeded water cluster geometry; integrals from this build's engine)",
         "config": {"workload": f"(H2O){n} seed {seed}, def2-SVP / def2-SVP-JKFIT, nbf={mol.nbf} naux={mol.naux} "
                                f"nocc={mol.nelectron // 2}; RI-HF to |dE| <= 1e-8 Eh, ||dD||/nbf <= 1e-5, "
