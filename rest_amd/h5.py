@@ -5,8 +5,9 @@ REST's formats (via the hdf5 crate):
   - guess file: dataset "init_guess" at the root, spin_channel × nbf² FP64 values
     (initial_guess_from_hdf5guess, src/initial_guess/mod.rs:75-87);
   - chkfile: group "scf" with "e_tot" (scalar), "mo_coeff" (spins back to
-    back, each the transposed nbf × nstate eigenvector matrix, i.e. row-major
-    [nstate][nbf]) and "mo_energy" (spins back to back)
+    back, each the data of the transposed column-major nbf × nstate eigenvector
+    matrix, i.e. C[ao][state] row-major — the bytes PySCF writes too) and
+    "mo_energy" (spins back to back)
     (save_chkfile, src/post_scf_analysis/mod.rs:47-103; read back by
     initial_guess_from_hdf5chk, src/initial_guess/mod.rs:89-128).
 
