@@ -692,11 +692,19 @@ def training_cluster(i: int, base_seed: int = 4096, nmin: int = 2, nmax: int = 2
     return wl.water_cluster(seed, n, oh_jitter=0.05, hoh_jitter_deg=5.0)
 
 
-def make_training_data(indices, base_seed: int = 4096, basis="def2-SVP", auxbasis="def2-SVP-JKFIT", log=print):
-    """Run the FP64 SCF (SAD start) on each training cluster and encode its converged density."""
+def make_training_data(indices, base_seed: int = 4096, basis="def2-SVP", auxbasis="def2-SVP-JKFIT", log=print,
+                       deadline: float = None, stats: dict = None):
+    """Run the FP64 SCF (SAD start) on each training cluster and encode its converged density.
+    deadline (time.time() value): stop before the next cluster once it has passed; stats receives
+    the number of clusters done."""
     Xs, Ts, Ms = [], [], []
     t0 = time.time()
+    done = 0
     for c, i in enumerate(indices):
+        if deadline is not None and time.time() > deadline:
+            log(f"deadline reached after {c} clusters")
+            break
+        done = c + 1
         xyz = training_cluster(i, base_seed)
         mol = ri.Molecule.water_cluster(xyz, basis, auxbasis)
         out, _ = rs.run_molecule(mol, guess="sad")
@@ -710,4 +718,9 @@ def make_training_data(indices, base_seed: int = 4096, basis="def2-SVP", auxbasi
         Ms.append(M)
         if (c + 1) % 16 == 0:
             log(f"  {c + 1}/{len(indices)} clusters, {sum(len(x) for x in Xs)} pairs ({time.time() - t0:.0f}s)")
+    if stats is not None:
+        stats["clusters_done"] = done
+    if not Xs:
+        return (np.zeros((0, N_IN), np.float32), np.zeros((0, MAXB, MAXB), np.float32),
+                np.zeros((0, MAXB, MAXB), np.float32))
     return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms)

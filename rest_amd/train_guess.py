@@ -1,20 +1,32 @@
 """Train the learned initial guess on densities this SCF produces, then compare
 it with SAD on held-out water clusters.
 
-    python -m rest_amd.train_guess --clusters 256 --epochs 40 --out rest_amd/models/learned_guess_h2o.npz
+    python -m rest_amd.train_guess --clusters 4096 --workers 6 --epochs 60 --out rest_amd/models/learned_guess_h2o.npz
+    python -m rest_amd.train_guess --eval-only --model rest_amd/models/learned_guess_h2o.npz \\
+        --test-n 64 --test-seeds 100 101 ... --report test.json
 
 Training clusters: learned.training_cluster(i), i < clusters: (H2O)n with n
 uniform in [2, 24], monomer O–H ±0.05 Å and H–O–H ±5°, def2-SVP /
-def2-SVP-JKFIT, converged with the FP64 device SCF from the SAD guess.
+def2-SVP-JKFIT, converged with the FP64 device SCF from the SAD guess.  With
+--workers W the clusters are split into W contiguous ranges, each converged by
+its own process on the same GPU (the per-cluster work — one-electron
+integrals on the host, the B build, small launch-bound SCF kernels — leaves
+the GPU mostly idle in a single process); every worker stops at the
+generation deadline and the training uses the clusters done by then.
 Held-out test: (H2O)n clusters from workload.water_cluster with seeds not used
-in training (--test-n, --test-seeds); for each, SCF iterations and guess
-energies with SAD and with the learned guess.
+in training (--test-n, --test-seeds); for each, the FP64 SCF from SAD and the
+FP64 SCF from the learned guess (learned.learned_scf: kept only when its
+energy is below that of SAD's first orbital density), with iterations, guess
+energies and final energies.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -37,7 +49,7 @@ def evaluate(net, n: int, seeds, prec_mixed: bool = False) -> list:
         m = ri.Molecule.water_cluster(wl.water_cluster(seed, n))
         S, T, V = m.int1e()
         H = T + V
-        store = jk.STORE_FP64 | (jk.STORE_FP32 if prec_mixed else 0)
+        store = jk.STORE_FP64 | (jk.STORE_FP32 | jk.STORE_BF16 if prec_mixed else 0)
         with jk.DeviceRI(m.nbf, m.naux) as r:
             m.build_rimatr(r, 1e-8, store)
             t0 = time.perf_counter()
@@ -45,51 +57,130 @@ def evaluate(net, n: int, seeds, prec_mixed: bool = False) -> list:
             e0, _, c1, _ = rs.guess_step(r, S, H, m.nuclear_repulsion(), (m.nelectron,), da + db)
             o_sad = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, None, rs.default_ctrl(), c_guess=c1)
             t1 = time.perf_counter()
-            g = lg.learned_guess(net, m, r, S, H, jk.PREC_FP32 if prec_mixed else jk.PREC_FP64)
-            ctrl = rs.default_ctrl(jk_prec=jk.PREC_FP32 if prec_mixed else jk.PREC_FP64)
-            o_l = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, None, ctrl, c_guess=g.c1)
+            ctrl = rs.default_ctrl()
+            if prec_mixed:
+                ctrl.jk_prec, ctrl.jk_prec_start = jk.PREC_BF16X3, jk.PREC_BF16X2
+            g = lg.learned_scf(net, m, r, S, H, ctrl)
             t2 = time.perf_counter()
+        o_l = g.outcome
         row = {"n": n, "seed": seed, "sad_iterations": o_sad.iterations, "sad_energy": o_sad.energy,
                "sad_guess_energy": e0, "sad_seconds": t1 - t0, "learned_kind": g.kind,
-               "learned_guess_energy": g.e_learned, "learned_electrons": g.electrons,
-               "learned_iterations": o_l.iterations, "learned_energy": o_l.energy, "learned_seconds": t2 - t1,
-               "dE": o_l.energy - o_sad.energy, "mixed": prec_mixed}
+               "learned_guess_energy": g.e_learned, "sad_d1_energy": g.e_sad, "learned_electrons": g.electrons,
+               "learned_iterations": o_l.iterations, "learned_fp64_iterations": o_l.fp64_iterations,
+               "learned_energy": o_l.energy, "learned_seconds": t2 - t1, "dE": o_l.energy - o_sad.energy,
+               "converged": bool(o_sad.converged and o_l.converged), "mixed": prec_mixed}
         log(json.dumps(row))
         rows.append(row)
     return rows
+
+
+def _generate_shard(a) -> int:
+    """Worker: converge clusters [i0, i1) until the deadline, save (X, T, M) to --shard-out."""
+    from . import learned as lg
+    i0, i1 = a.shard
+    stats = {}
+    X, T, M = lg.make_training_data(range(i0, i1), a.base_seed, log=lambda m: log(f"[worker {i0}] {m}"),
+                                    deadline=a.deadline, stats=stats)
+    np.savez(a.shard_out, X=X, T=T, M=M.astype(np.uint8), done=np.int64(stats.get("clusters_done", 0)))
+    log(f"[worker {i0}] saved {stats.get('clusters_done', 0)} clusters, {len(X)} pairs")
+    return 0
+
+
+def _generate_parallel(a):
+    """Spawn the workers (this process has not touched the GPU), wait, and join their shards."""
+    tmp = Path(tempfile.mkdtemp(prefix="lg_shards_"))
+    deadline = time.time() + a.gen_seconds
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(max(1, a.cpu_threads // a.workers))
+    edges = np.linspace(0, a.clusters, a.workers + 1).astype(int)
+    procs = []
+    for w in range(a.workers):
+        out = tmp / f"shard{w}.npz"
+        cmd = [sys.executable, "-u", "-m", "rest_amd.train_guess", "--shard", str(edges[w]), str(edges[w + 1]),
+               "--shard-out", str(out), "--deadline", repr(deadline), "--base-seed", str(a.base_seed),
+               "--out", str(out)]
+        procs.append((subprocess.Popen(cmd, env=env), out))
+    Xs, Ts, Ms, done = [], [], [], 0
+    for p, out in procs:
+        rc = p.wait()
+        if rc != 0:
+            raise RuntimeError(f"data worker for {out.name} exited with {rc}")
+        with np.load(out, allow_pickle=False) as z:
+            Xs.append(z["X"])
+            Ts.append(z["T"])
+            Ms.append(z["M"].astype(np.float32))
+            done += int(z["done"])
+        out.unlink()
+    tmp.rmdir()
+    return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms), done
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--clusters", type=int, default=256)
     ap.add_argument("--base-seed", type=int, default=4096)
+    ap.add_argument("--workers", type=int, default=1, help="data-generation processes sharing the GPU")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="host threads split among the workers")
+    ap.add_argument("--gen-seconds", type=float, default=1e9, help="data-generation deadline (seconds)")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--hidden", type=int, default=256)
     ap.add_argument("--layers", type=int, default=3)
-    ap.add_argument("--out", type=Path, required=True)
+    ap.add_argument("--out", type=Path, default=None)
+    ap.add_argument("--eval-only", action="store_true", help="evaluate --model on the test clusters")
+    ap.add_argument("--model", type=Path, default=None)
     ap.add_argument("--test-n", type=int, default=64)
     ap.add_argument("--test-seeds", type=int, nargs="*", default=[2])
+    ap.add_argument("--mixed-eval", action="store_true", help="also evaluate the mixed-precision SCF")
     ap.add_argument("--report", type=Path, default=None)
+    ap.add_argument("--shard", type=int, nargs=2, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--shard-out", type=Path, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--deadline", type=float, default=None, help=argparse.SUPPRESS)
     a = ap.parse_args(argv)
+    if a.shard is not None:
+        return _generate_shard(a)
 
-    from . import learned as lg
+    if a.eval_only:
+        from . import learned as lg
+        net = lg.Net.load(a.model)
+        rows = evaluate(net, a.test_n, a.test_seeds)
+        if a.mixed_eval:
+            rows += evaluate(net, a.test_n, a.test_seeds, prec_mixed=True)
+        if a.report:
+            a.report.write_text(json.dumps({"model": str(a.model), "meta": getattr(net, "meta", {}), "test": rows},
+                                           indent=1))
+        return 0
+
+    if a.out is None:
+        ap.error("--out is required for training")
     t0 = time.time()
-    log(f"generating training data: {a.clusters} clusters")
-    X, T, M = lg.make_training_data(range(a.clusters), a.base_seed, log=log)
-    log(f"{len(X)} pair samples in {time.time() - t0:.0f}s; training {a.layers}x{a.hidden} SiLU MLP")
+    log(f"generating training data: {a.clusters} clusters, {a.workers} worker(s)")
+    if a.workers > 1:
+        X, T, M, done = _generate_parallel(a)
+    else:
+        from . import learned as lg
+        stats = {}
+        X, T, M = lg.make_training_data(range(a.clusters), a.base_seed, log=log,
+                                        deadline=time.time() + a.gen_seconds, stats=stats)
+        done = stats["clusters_done"]
+    t_gen = time.time() - t0
+    from . import learned as lg
+    log(f"{done} clusters, {len(X)} pair samples in {t_gen:.0f}s; training {a.layers}x{a.hidden} SiLU MLP")
     net = lg.Net(lg.N_IN, lg.N_OUT, a.hidden, a.layers, max_batch=a.batch, seed=0)
     hist = lg.train(net, X, T, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
-    meta = {"clusters": a.clusters, "base_seed": a.base_seed, "samples": int(len(X)), "epochs": a.epochs,
-            "batch": a.batch, "lr": a.lr, "train_loss": hist["train"][-1], "val_loss": hist["val"][-1],
-            "train_seconds": time.time() - t0}
+    meta = {"clusters": int(done), "clusters_requested": a.clusters, "base_seed": a.base_seed,
+            "samples": int(len(X)), "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "workers": a.workers,
+            "train_loss": hist["train"][-1], "val_loss": hist["val"][-1], "generation_seconds": t_gen,
+            "train_seconds": time.time() - t0 - t_gen}
     a.out.parent.mkdir(parents=True, exist_ok=True)
     net.save(a.out, meta)
     log(f"saved {a.out}: {json.dumps(meta)}")
     rows = []
     if a.test_seeds:
-        rows = evaluate(net, a.test_n, a.test_seeds) + evaluate(net, a.test_n, a.test_seeds, prec_mixed=True)
+        rows = evaluate(net, a.test_n, a.test_seeds)
+        if a.mixed_eval:
+            rows += evaluate(net, a.test_n, a.test_seeds, prec_mixed=True)
     if a.report:
         a.report.write_text(json.dumps({"meta": meta, "history": hist, "test": rows}, indent=1))
     return 0
