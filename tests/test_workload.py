@@ -29,3 +29,30 @@ def test_shapes():
     assert abs(wl.jk_bytes(wl.H2O_64, 8) - 136.59e9) / 136.59e9 < 1e-3
     assert abs(wl.k_flops(wl.H2O_64)["total"] - 1.638e13) / 1.638e13 < 1e-3
     assert abs(wl.k_flops(wl.H2O_16)["total"] - 6.404e10) / 6.404e10 < 1e-3
+
+
+def _read_xyz(name):
+    from pathlib import Path
+    lines = (Path(__file__).resolve().parent / "golden" / name).read_text().splitlines()[2:]
+    return [l.split()[0] for l in lines], np.array([[float(v) for v in l.split()[1:]] for l in lines])
+
+
+def test_benchmark_geometries_match_fixtures():
+    """The (H2O)16 seed-1 and (H2O)64 seed-2 clusters the benchmarks run are the committed .xyz fixtures;
+    every monomer keeps the reference geometry (O–H 0.95777559 Å, H–O–H 104.51°) and no two oxygens
+    are closer than 2.5 Å."""
+    for name, seed, n in (("h2o16_seed1.xyz", 1, 16), ("h2o64_seed2.xyz", 2, 64)):
+        sym, ref = _read_xyz(name)
+        xyz = wl.water_cluster(seed, n)
+        assert sym == ["O", "H", "H"] * n
+        np.testing.assert_allclose(xyz, ref, rtol=0, atol=1e-11)
+        o, h1, h2 = xyz[0::3], xyz[1::3], xyz[2::3]
+        d1, d2 = np.linalg.norm(h1 - o, axis=1), np.linalg.norm(h2 - o, axis=1)
+        np.testing.assert_allclose(np.r_[d1, d2], np.hypot(0.7572, 0.5865), atol=1e-12)
+        cos = np.sum((h1 - o) * (h2 - o), axis=1) / (d1 * d2)
+        np.testing.assert_allclose(np.degrees(np.arccos(cos)), np.degrees(2 * np.arctan2(0.7572, 0.5865)),
+                                   atol=1e-9)
+        doo = np.linalg.norm(o[:, None] - o[None], axis=2) + np.eye(n) * 1e9
+        assert doo.min() >= 2.5
+    sym, mono = _read_xyz("h2o_monomer.xyz")
+    assert sym == ["O", "H", "H"] and np.allclose(mono[1], [0.7572, 0.5865, 0.0])
