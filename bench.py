@@ -124,10 +124,10 @@ def main():
     return main_jk(args)
 
 
-def roofline_from_detail(detail: dict, bf16_products: int = 6):
+def roofline_from_detail(detail: dict):
     """Dominant kernel of the timed region and its achieved rate from the library's per-launch work.
-    BF16-plane launches execute bf16_products MFMA products per algorithmic product: their rate is
-    the executed MFMA rate against the BF16 peak."""
+    BF16-plane launches execute 3 (BF16X2) or 6 (BF16X3) MFMA products per algorithmic product: their
+    rate is the executed MFMA rate against the BF16 peak."""
     best = None
     for (prec, phase), (ms, n, work) in detail.items():
         if phase in PHASE_KERNEL and n and (best is None or ms > best[2]):
@@ -137,10 +137,10 @@ def roofline_from_detail(detail: dict, bf16_products: int = 6):
     if phase.startswith("j_"):
         ach, peak, unit, bound = work / sec / 1e9, HBM_PEAK_GBS, "GB/s", "hbm"
     else:
-        mult = bf16_products if prec == "bf16" else 1
-        ach, peak, unit, bound = mult * work / sec / 1e12, MFMA_PEAK_TFS[prec], "TFLOP/s", "mfma"
-    if prec == "bf16" and not phase.startswith("j_"):
-        kernel = {"k_half": "k_bf_khalf", "k_syrk": "k_bf_ksyrk"}[phase] + f"<{3 if bf16_products == 6 else 2}>"
+        mult = BF16_PRODUCTS.get(prec, 1)
+        ach, peak, unit, bound = mult * work / sec / 1e12, MFMA_PEAK_TFS["bf16" if mult > 1 else prec], "TFLOP/s", "mfma"
+    if prec in BF16_PRODUCTS and not phase.startswith("j_"):
+        kernel = {"k_half": "k_bf_khalf", "k_syrk": "k_bf_ksyrk"}[phase] + f"<{prec[-1]}>"
     else:
         kernel = PHASE_KERNEL[phase] + ("<double>" if prec == "fp64" else "<float>")
     return {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
@@ -219,7 +219,7 @@ def main_scf(args):
     r.profile_enable(False)
     op, ip, om, im = runs[-1]
     t_plain, t_mixed = float(np.median(tp)), float(np.median(tm))
-    roofline = roofline_from_detail(detail_prof, BF16_PRODUCTS.get(args.mixed_prec, 6))
+    roofline = roofline_from_detail(detail_prof)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -170,8 +170,8 @@ int rjk_vk_upper_with_ri_v_use_dm_only_sync(rjk_ctx* ctx, const double* dm,
  * launches[5] and resets the counters. */
 int rjk_profile_enable(rjk_ctx* ctx, int on);
 int rjk_profile_read(rjk_ctx* ctx, double* ms, uint64_t* launches);
-/* The same counters split by arithmetic, [3][5] row-major (row 0 FP64, row 1
- * FP32, row 2 BF16 planes), plus the algorithmic work of those launches: bytes for the J
+/* The same counters split by arithmetic, [4][5] row-major (row = the
+ * RJK_PREC_* value: FP64, FP32, BF16X2, BF16X3), plus the algorithmic work of those launches: bytes for the J
  * phases (npair·naux·s_B + 8·nspin·npair per pass), FLOP for the K phases
  * (2·nbf²·nk·P per half-transform, nbf(nbf+1)·nk·P per SYRK), 0 for phase 4.
  * Resets the counters like rjk_profile_read. */

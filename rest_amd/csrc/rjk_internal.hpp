@@ -21,10 +21,10 @@ constexpr int kTileK = 16;           // k-block of the K tiles
 constexpr int kColPad = 64;          // occupied-column padding of M (all spins)
 constexpr int kBfColGran = 32;       // BF16 K: per-spin column granule (one MFMA k-block)
 constexpr int kBfStoredPlanes = 3;   // BF16 planes kept per B element (RJK_STORE_BF16)
-constexpr int kNumPrec = 3;          // profile rows: FP64, FP32, BF16 planes
+constexpr int kNumPrec = 4;          // profile rows: one per RJK_PREC_* value
 
 inline bool is_bf16(int prec) { return prec == RJK_PREC_BF16X2 || prec == RJK_PREC_BF16X3; }
-inline int prof_row(int prec) { return prec == RJK_PREC_FP64 ? 0 : (is_bf16(prec) ? 2 : 1); }
+inline int prof_row(int prec) { return prec >= 0 && prec < kNumPrec ? prec : 0; }
 
 inline size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -53,7 +53,7 @@ struct rjk_ctx {
     std::string err;
     // phase timing
     bool prof = false;
-    double prof_ms[rjk::kNumPrec][rjk::kNumPhases] = {};     // [FP64, FP32, BF16][phase]
+    double prof_ms[rjk::kNumPrec][rjk::kNumPhases] = {};     // [RJK_PREC_*][phase]
     uint64_t prof_n[rjk::kNumPrec][rjk::kNumPhases] = {};
     double prof_work[rjk::kNumPrec][rjk::kNumPhases] = {};   // algorithmic bytes (J) or FLOP (K)
     struct EvPair { hipEvent_t a, b; int phase; int prec; double work; };

@@ -313,13 +313,13 @@ class DeviceRI:
         return {name: (ms[k], int(n[k])) for k, name in enumerate(self.PHASES)}
 
     def profile_read_detail(self) -> dict:
-        """{(prec, phase): (ms, launches, algorithmic work)} with prec "fp64" / "fp32" / "bf16"."""
-        ms = (C.c_double * 15)()
-        n = (C.c_uint64 * 15)()
-        w = (C.c_double * 15)()
+        """{(prec, phase): (ms, launches, algorithmic work)} with prec "fp64" / "fp32" / "bf16x2" / "bf16x3"."""
+        ms = (C.c_double * 20)()
+        n = (C.c_uint64 * 20)()
+        w = (C.c_double * 20)()
         self._check(self.lib.rjk_profile_read_detail(self.h, ms, n, w))
         return {(p, name): (ms[5 * pi + k], int(n[5 * pi + k]), w[5 * pi + k])
-                for pi, p in enumerate(("fp64", "fp32", "bf16")) for k, name in enumerate(self.PHASES)}
+                for pi, p in enumerate(("fp64", "fp32", "bf16x2", "bf16x3")) for k, name in enumerate(self.PHASES)}
 
     def copy_dev(self, dst_ptr: int, src_ptr: int, nbytes: int):
         self._check(self.lib.rjk_copy_dev(self.h, C.c_void_p(dst_ptr), C.c_void_p(src_ptr), int(nbytes)))

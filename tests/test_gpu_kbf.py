@@ -145,3 +145,27 @@ def test_bf16_j_runs_on_the_stored_copy_and_error_paths():
         with pytest.raises(jk.RJKError) as e:
             ri.vk_occ([c], [5], [np.full(6, 2.0)], 1, 1.0, prec=7)
         assert e.value.code == 1
+
+
+def test_profile_rows_per_precision():
+    """rjk_profile_read_detail keeps one row per RJK_PREC_* value; K work is the algorithmic FLOP of the
+    padded columns (2·nbf²·nk·naux half-transform); J at a BF16 precision runs and is counted as FP32."""
+    nbf, naux, nocc = 130, 40, 17
+    say(f"profile rows nbf={nbf} naux={naux}")
+    c = orc.random_orthonormal(5, nbf, nocc)
+    occ = np.full(nocc, 2.0)
+    with jk.DeviceRI(nbf, naux) as ri:
+        ri.fill_synthetic(21, STORE)
+        ri.profile_enable(True)
+        ri.profile_read_detail()
+        for p in (jk.PREC_FP64, jk.PREC_BF16X3, jk.PREC_BF16X2):
+            ri.vk_occ([c], [nocc - 1], [occ], 1, 1.0, prec=p)
+        ri.vj([c @ c.T], 1, 1.0, prec=jk.PREC_BF16X3)
+        d = ri.profile_read_detail()
+    nk = 64   # 17 columns, padded to the column granule and then to 64
+    for name in ("fp64", "bf16x3", "bf16x2"):
+        ms, n, work = d[(name, "k_half")]
+        assert n == 1 and ms > 0 and work == pytest.approx(2.0 * nbf * nbf * nk * naux), name
+        assert d[(name, "k_syrk")][1] == 1
+    assert d[("fp32", "j_gamma")][1] == 1 and d[("fp32", "k_half")][1] == 0
+    assert d[("fp64", "j_gamma")][1] == 0
