@@ -423,6 +423,15 @@ def main_jk(args):
     phases = ri.profile_read()
     ri.profile_enable(False)
     assert np.isfinite(vj_dev.cpu().numpy()).all() and np.isfinite(vk_dev.cpu().numpy()).all()
+    errors = None
+    if prec != jk.PREC_FP64:   # J and K of the same step at FP64 (outside the timed region)
+        vj_lo, vk_lo = vj_dev.cpu().numpy(), vk_dev.cpu().numpy()
+        ri.vj_dev(d_dev.data_ptr(), 1, 1.0, vj_dev.data_ptr(), prec=jk.PREC_FP64)
+        ri.vk_occ_dev(c_dev.data_ptr(), nocc, [nocc - 1], occ_dev.data_ptr(), 1, 1.0, vk_dev.data_ptr(),
+                      prec=jk.PREC_FP64)
+        vj64, vk64 = vj_dev.cpu().numpy(), vk_dev.cpu().numpy()
+        errors = {"j_max_rel_err_vs_fp64": float(np.abs(vj_lo - vj64).max() / np.abs(vj64).max()),
+                  "k_max_rel_err_vs_fp64": float(np.abs(vk_lo - vk64).max() / np.abs(vk64).max())}
 
     ms_step = ms_total / args.steps
     kf = wl.k_flops(shape)
@@ -489,7 +498,7 @@ def main_jk(args):
                    "nbf": nbf, "naux": naux, "nocc": nocc, "nspin": 1},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "detail": detail,
+        "detail": {**detail, "errors_vs_fp64": errors},
     }
     print(json.dumps(out))
     ri.close()
