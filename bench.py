@@ -59,6 +59,8 @@ def parse():
                          "bf16x3 / bf16x2: K on three / two BF16 planes of B (6 / 3 MFMA products), J on the FP32 copy")
     ap.add_argument("--mixed-prec", default="bf16x3", choices=["fp32", "bf16x3"],
                     help="scf mode: J/K precision of the mixed SCF before its switch to FP64")
+    ap.add_argument("--pseudo-diag-err", type=float, default=0.0,
+                    help="scf mode: low-precision iterations with max|e| below this pseudo-diagonalise (0: never)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -190,9 +192,11 @@ def main_scf(args):
         ctrl = rs.default_ctrl(jk_prec=mixed_prec)
         if args.mixed_prec.startswith("bf16"):
             ctrl.jk_prec_start = jk.PREC_BF16X2        # BF16X2 until max|e| < 1e-3
+        ctrl.pseudo_diag_err = args.pseudo_diag_err
         g = lg.learned_scf(net, mol, r, S, H, ctrl)
         return g.outcome, {"guess": g.kind, "e_learned": g.e_learned, "e_sad": g.e_sad, "seconds": g.seconds,
-                           "precisions": [int(x["prec"]) for x in g.outcome.log]}
+                           "precisions": [int(x["prec"]) for x in g.outcome.log],
+                           "diag_ms": [round(float(x["diag_ms"]), 2) for x in g.outcome.log]}
 
     for _ in range(args.warmup):
         plain()

@@ -48,6 +48,15 @@ typedef struct rscf_ctrl {
     double start_switch_err; /* 1e-3 */
     double guess_energy_max; /* return before the first iteration when the guess Fock build gives
                               * E(guess) >= this (result.guess_rejected = 1); HUGE_VAL (default): never */
+    double pseudo_diag_err;  /* iterations with low-precision J/K and max|e| below this rotate the
+                              * previous orbitals (pseudo-diagonalisation) instead of calling the
+                              * eigensolver; full FP64 iterations diagonalise exactly; 0 (default): never */
+    int jk_prec_increment;   /* -1 (default): every FP64 build is a full one.  Else, after the switch
+                              * from low-precision J/K to FP64, the first FP64 build is a reference
+                              * and later ones add J/K of D − D_ref at this precision (RJK_PREC_FP32,
+                              * _BF16X2 or _BF16X3); from the reference on every iteration
+                              * pseudo-diagonalises, and the outputs come from one exact
+                              * diagonalisation at the end */
 } rscf_ctrl;
 
 /* What DIIS keeps at the precision switch.  RESTART: the stored low-precision
@@ -70,6 +79,7 @@ typedef struct rscf_result {
     double jk_seconds, diag_seconds, diis_seconds;
     double guess_energy;     /* energy of the guess Fock build (the given density or orbitals) */
     int guess_rejected;      /* 1: guess_energy >= ctrl.guess_energy_max and no iteration ran */
+    int incremental_iterations; /* FP64 iterations whose J/K were increments on the reference build */
 } rscf_result;
 
 void rscf_default_ctrl(rscf_ctrl* ctrl);

@@ -99,6 +99,76 @@ __global__ void k_axpy_inplace(double* __restrict__ y, const double* __restrict_
         y[t] += a * x[t];
 }
 
+// Columns of ΔD = occ·(A Aᵀ − R Rᵀ) over the first nocc columns of A and R (ld n):
+// P = A_i + R_i, M = A_i − R_i, u = a P + b M, v = a P − b M with a b = ½ and
+// a²|P|² = b²|M|², so Σ_i (u uᵀ − v vᵀ) = Σ_i (P Mᵀ + M Pᵀ) = 2 (A Aᵀ − R Rᵀ) and
+// |u|², |v|² <= 2|P||M|: K of [u | v] with weights ±occ/2 at a low precision errs in
+// proportion to |A − R|, not to |A|.  u → UV[:, i], v → UV[:, nocc + i]; one block per column.
+__global__ void k_inc_columns(const double* __restrict__ A, const double* __restrict__ R, size_t n, size_t nocc,
+                              double* __restrict__ UV) {
+    __shared__ double sp[256], sm[256];
+    const size_t i = blockIdx.x;
+    const double* a_col = A + i * n;
+    const double* r_col = R + i * n;
+    double pp = 0.0, mm = 0.0;
+    for (size_t r = threadIdx.x; r < n; r += blockDim.x) {
+        const double p = a_col[r] + r_col[r], m = a_col[r] - r_col[r];
+        pp += p * p;
+        mm += m * m;
+    }
+    sp[threadIdx.x] = pp;
+    sm[threadIdx.x] = mm;
+    __syncthreads();
+    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sp[threadIdx.x] += sp[threadIdx.x + w];
+            sm[threadIdx.x] += sm[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    const double pn = sqrt(sp[0]), mn = sqrt(sm[0]);
+    double ca = 0.0, cb = 0.0;   // P = 0 or M = 0: the column adds nothing to P Mᵀ + M Pᵀ
+    if (pn > 0.0 && mn > 0.0) {
+        ca = sqrt(mn / (2.0 * pn));
+        cb = 0.5 / ca;
+    }
+    for (size_t r = threadIdx.x; r < n; r += blockDim.x) {
+        const double p = a_col[r] + r_col[r], m = a_col[r] - r_col[r];
+        UV[r + i * n] = ca * p + cb * m;
+        UV[r + (nocc + i) * n] = ca * p - cb * m;
+    }
+}
+
+// out = A − B
+__global__ void k_sub(const double* __restrict__ A, const double* __restrict__ B, size_t n2, double* __restrict__ out) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
+        out[t] = A[t] - B[t];
+}
+
+// U = I + X with X[a][i] = F_ai / (ε_i − ε_a), X[i][a] = −X[a][i] for i < nocc <= a
+// (first-order rotation of the occupied orbitals into the virtual space); ε = diag(Fmo),
+// |ε_i − ε_a| clamped below by gmin.  Column-major n × n.
+__global__ void k_pseudo_rot(const double* __restrict__ Fmo, size_t n, size_t nocc, double gmin,
+                             double* __restrict__ U) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = t % n, c = t / n;
+        double v = r == c ? 1.0 : 0.0;
+        if (c < nocc && r >= nocc) {          // X[a = r][i = c]
+            const double den = fmin(Fmo[c * (n + 1)] - Fmo[r * (n + 1)], -gmin);
+            v = Fmo[r + c * n] / den;
+        } else if (r < nocc && c >= nocc) {   // X[i = r][a = c] = −X[a][i]
+            const double den = fmin(Fmo[r * (n + 1)] - Fmo[c * (n + 1)], -gmin);
+            v = -Fmo[c + r * n] / den;
+        }
+        U[t] = v;
+    }
+}
+
+__global__ void k_diag_copy(const double* __restrict__ M, size_t n, double* __restrict__ d) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = M[i * (n + 1)];
+}
+
 // y = x − y
 __global__ void k_delta_inplace(const double* __restrict__ x, double* __restrict__ y, size_t n2) {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
@@ -251,6 +321,8 @@ extern "C" void rscf_default_ctrl(rscf_ctrl* c) {
     c->jk_prec_start = -1;
     c->start_switch_err = 1e-3;
     c->guess_energy_max = HUGE_VAL;
+    c->pseudo_diag_err = 0.0;
+    c->jk_prec_increment = -1;
 }
 
 namespace {
@@ -291,7 +363,9 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64, FP32, BF16X2 or BF16X3");
     if (ctrl.jk_prec_start != -1 && ctrl.jk_prec_start != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec_start))
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec_start must be -1, FP32, BF16X2 or BF16X3");
-    for (int pr : {ctrl.jk_prec, ctrl.jk_prec_start}) {
+    if (ctrl.jk_prec_increment != -1 && ctrl.jk_prec_increment != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec_increment))
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec_increment must be -1, FP32, BF16X2 or BF16X3");
+    for (int pr : {ctrl.jk_prec, ctrl.jk_prec_start, ctrl.jk_prec_increment}) {
         if (pr == RJK_PREC_FP32 && !ctx->b32)
             return rjk::fail(ctx, RJK_ERR_STATE,
                              std::string(who) + ": mixed precision needs the FP32 copy of B (RJK_STORE_FP32)");
@@ -341,6 +415,24 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     double* ering = mem.get(ctx, (size_t)nd * ev_len, &rc);
     double* fring = mem.get(ctx, (size_t)nd * ev_len, &rc);
     double* dring = mem.get(ctx, (size_t)nd * ev_len, &rc);   // the densities of the stored Fock matrices
+    // incremental builds after the FP64 switch: the reference build's total density, J and
+    // per-spin K and occupied orbitals; per spin the [u | v] columns of ΔD and their weights
+    const bool inc_enabled = ctrl.jk_prec_increment >= 0;
+    double *Dref = nullptr, *vjref = nullptr, *Dinc = nullptr;
+    double *Cref[2] = {nullptr, nullptr}, *vkref[2] = {nullptr, nullptr}, *UV[2] = {nullptr, nullptr},
+           *occ2[2] = {nullptr, nullptr};
+    if (inc_enabled) {
+        Dref = mem.get(ctx, n2, &rc);
+        Dinc = mem.get(ctx, n2, &rc);
+        vjref = mem.get(ctx, npair, &rc);
+        for (int s = 0; s < nspin; ++s) {
+            const size_t no = std::max<size_t>(nocc[s], 1);
+            Cref[s] = mem.get(ctx, n * no, &rc);
+            UV[s] = mem.get(ctx, 2 * n * no, &rc);
+            occ2[s] = mem.get(ctx, 2 * no, &rc);
+            vkref[s] = mem.get(ctx, npair, &rc);
+        }
+    }
     if (rc) return rc;
     if (ctrl.switch_mode != RSCF_SWITCH_RESTART && ctrl.switch_mode != RSCF_SWITCH_SHIFT)
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": switch_mode must be RSCF_SWITCH_RESTART or _SHIFT");
@@ -350,7 +442,15 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         std::vector<double> occ(n, 0.0);
         for (size_t i = 0; i < nocc[s]; ++i) occ[i] = occ_val;
         SCF_HIP(hipMemcpyAsync(occd[s], occ.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
-        SCF_HIP(hipStreamSynchronize(st));   // occ is a host temporary
+        if (inc_enabled && nocc[s] > 0) {
+            std::vector<double> w2(2 * nocc[s]);
+            for (size_t i = 0; i < nocc[s]; ++i) {
+                w2[i] = 0.5 * occ_val;
+                w2[nocc[s] + i] = -0.5 * occ_val;
+            }
+            SCF_HIP(hipMemcpyAsync(occ2[s], w2.data(), w2.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        }
+        SCF_HIP(hipStreamSynchronize(st));   // occ and w2 are host temporaries
     }
     const dim3 g2((unsigned)((n2 + 255) / 256)), b256(256);
     const double one = 1.0, zero = 0.0;
@@ -439,6 +539,44 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         t_diag += now() - t0;
         return RJK_OK;
     };
+    // Pseudo-diagonalisation of Fm for spin s (Stewart, Császár & Pulay, J. Comput. Chem. 3, 227
+    // (1982)): the current orbitals A[s] turn F into Fmo = Aᵀ F A; the occupied orbitals take the
+    // first-order rotation U = I + X toward the virtual space; A U is made S-orthonormal again by
+    // the Cholesky factor of (A U)ᵀ S (A U) = Uᵀ U; ε ≈ diag(Fmo).  Four GEMMs, one Cholesky and one
+    // triangular solve instead of a full eigensolver; used where the Fock matrix carries
+    // low-precision J/K (the FP64 iterations diagonalise exactly).
+    auto pseudo_diagonalise = [&](const double* Fm, int s) -> int {
+        const double t0 = now();
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, Fm, (rocblas_int)n, A[s], (rocblas_int)n, &zero, T1,
+                               (rocblas_int)n));
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, A[s], (rocblas_int)n, T1, (rocblas_int)n, &zero, T2,
+                               (rocblas_int)n));
+        hipLaunchKernelGGL(k_diag_copy, dim3((unsigned)((n + 255) / 256)), b256, 0, st, T2, n, W[s]);
+        SCF_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_pseudo_rot, dim3(kRedBlocks), b256, 0, st, T2, n, nocc[s], 0.05, T1);
+        SCF_HIP(hipGetLastError());
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, A[s], (rocblas_int)n, T1, (rocblas_int)n, &zero, Bw,
+                               (rocblas_int)n));
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                               (rocblas_int)n, &one, T1, (rocblas_int)n, T1, (rocblas_int)n, &zero, T2,
+                               (rocblas_int)n));
+        SCF_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, (rocblas_int)n, T2, (rocblas_int)n, (rocblas_int*)info_d));
+        SCF_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                               rocblas_diagonal_non_unit, (rocblas_int)n, (rocblas_int)n, &one, T2, (rocblas_int)n, Bw,
+                               (rocblas_int)n));
+        SCF_HIP(hipMemcpyAsync(A[s], Bw, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        int info = 0;
+        SCF_HIP(hipMemcpyAsync(&info, info_d, sizeof(int), hipMemcpyDeviceToHost, st));
+        SCF_HIP(hipMemcpyAsync(eps[s].data(), W[s], n * sizeof(double), hipMemcpyDeviceToHost, st));
+        SCF_HIP(hipStreamSynchronize(st));
+        if (info != 0) return rjk::fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dpotrf (pseudo-diagonalisation) info = " +
+                                                                   std::to_string(info));
+        t_diag += now() - t0;
+        return RJK_OK;
+    };
     // D_s = occ · C_occ C_occᵀ from A[s] (generate_density_matrix, REST :441-478)
     auto density = [&](int s, double* Dout) -> int {
         if (nocc[s] == 0) {
@@ -454,6 +592,13 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     // F_s = H + J[Σ_s D_s] + sK·K[D_s]; K by the occupied route when the densities
     // are those of the current orbitals A, else by the density-only route.
     // E = E_nuc + ½ Σ_s Σ D_s (H + F_s)   (REST :1715-1782, :1673-1713)
+    // Incremental form (inc_armed: after the switch to FP64 with jk_prec_increment set): the
+    // first FP64 build by the occupied route becomes the reference (D_ref, C_ref, J_ref, K_ref);
+    // later FP64 builds are J_ref + J(D − D_ref) and K_ref + K(ΔD) with the increments at
+    // jk_prec_increment, K(ΔD) through the [u | v] columns of k_inc_columns, so their rounding
+    // error scales with |D − D_ref| (Almlöf, Faegri & Korsell, J. Comput. Chem. 3, 385 (1982)).
+    bool inc_armed = false, have_ref = false;
+    int n_inc = 0;
     auto fock = [&](double* const* Dsrc, bool occ_route, int prec, double* E) -> int {
         const double t0 = now();
         const double* Djk = Dsrc[0];
@@ -462,21 +607,56 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             SCF_HIP(hipGetLastError());
             Djk = Dt;
         }
-        SCF_RC(rjk_vj_upper_dev(ctx, Djk, 1, 1.0, prec, 0, vj));
-        for (int s = 0; s < nspin; ++s) {
-            if (nocc[s] == 0) {
-                hipLaunchKernelGGL(k_fill, dim3((unsigned)((npair + 255) / 256)), b256, 0, st, vk[s], npair, 0.0);
+        const bool inc_route = inc_armed && prec == RJK_PREC_FP64 && occ_route;
+        if (inc_route && have_ref) {
+            const int pinc = ctrl.jk_prec_increment;
+            hipLaunchKernelGGL(k_sub, dim3(kRedBlocks), b256, 0, st, Djk, Dref, n2, Dinc);
+            SCF_HIP(hipGetLastError());
+            SCF_RC(rjk_vj_upper_dev(ctx, Dinc, 1, 1.0, pinc, 0, vj));
+            hipLaunchKernelGGL(k_axpy_inplace, dim3(kRedBlocks), b256, 0, st, vj, vjref, 1.0, npair);
+            SCF_HIP(hipGetLastError());
+            for (int s = 0; s < nspin; ++s) {
+                if (nocc[s] == 0) {
+                    hipLaunchKernelGGL(k_fill, dim3((unsigned)((npair + 255) / 256)), b256, 0, st, vk[s], npair, 0.0);
+                    SCF_HIP(hipGetLastError());
+                    continue;
+                }
+                hipLaunchKernelGGL(k_inc_columns, dim3((unsigned)nocc[s]), b256, 0, st, A[s], Cref[s], n, nocc[s],
+                                   UV[s]);
                 SCF_HIP(hipGetLastError());
-            } else if (occ_route) {
-                const size_t homo[2] = {nocc[s] - 1, 0};
-                SCF_RC(rjk_vk_upper_occ_dev(ctx, A[s], n, homo, occd[s], 1, sk, prec, vk[s]));
-            } else {
-                std::vector<double> dh(n2), vkh(npair);
-                SCF_HIP(hipMemcpyAsync(dh.data(), Dsrc[s], n2 * sizeof(double), hipMemcpyDeviceToHost, st));
-                SCF_HIP(hipStreamSynchronize(st));
-                SCF_RC(rjk_vk_upper_with_rimatr_use_dm_only_sync(ctx, dh.data(), 1, sk, prec, vkh.data()));
-                SCF_HIP(hipMemcpyAsync(vk[s], vkh.data(), npair * sizeof(double), hipMemcpyHostToDevice, st));
-                SCF_HIP(hipStreamSynchronize(st));   // vkh is a host temporary
+                const size_t homo[2] = {2 * nocc[s] - 1, 0};
+                SCF_RC(rjk_vk_upper_occ_dev(ctx, UV[s], 2 * nocc[s], homo, occ2[s], 1, sk, pinc, vk[s]));
+                hipLaunchKernelGGL(k_axpy_inplace, dim3(kRedBlocks), b256, 0, st, vk[s], vkref[s], 1.0, npair);
+                SCF_HIP(hipGetLastError());
+            }
+            ++n_inc;
+        } else {
+            SCF_RC(rjk_vj_upper_dev(ctx, Djk, 1, 1.0, prec, 0, vj));
+            for (int s = 0; s < nspin; ++s) {
+                if (nocc[s] == 0) {
+                    hipLaunchKernelGGL(k_fill, dim3((unsigned)((npair + 255) / 256)), b256, 0, st, vk[s], npair, 0.0);
+                    SCF_HIP(hipGetLastError());
+                } else if (occ_route) {
+                    const size_t homo[2] = {nocc[s] - 1, 0};
+                    SCF_RC(rjk_vk_upper_occ_dev(ctx, A[s], n, homo, occd[s], 1, sk, prec, vk[s]));
+                } else {
+                    std::vector<double> dh(n2), vkh(npair);
+                    SCF_HIP(hipMemcpyAsync(dh.data(), Dsrc[s], n2 * sizeof(double), hipMemcpyDeviceToHost, st));
+                    SCF_HIP(hipStreamSynchronize(st));
+                    SCF_RC(rjk_vk_upper_with_rimatr_use_dm_only_sync(ctx, dh.data(), 1, sk, prec, vkh.data()));
+                    SCF_HIP(hipMemcpyAsync(vk[s], vkh.data(), npair * sizeof(double), hipMemcpyHostToDevice, st));
+                    SCF_HIP(hipStreamSynchronize(st));   // vkh is a host temporary
+                }
+            }
+            if (inc_route) {   // this FP64 build is the reference of the increments that follow
+                SCF_HIP(hipMemcpyAsync(Dref, Djk, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+                SCF_HIP(hipMemcpyAsync(vjref, vj, npair * sizeof(double), hipMemcpyDeviceToDevice, st));
+                for (int s = 0; s < nspin; ++s) {
+                    if (nocc[s])
+                        SCF_HIP(hipMemcpyAsync(Cref[s], A[s], n * nocc[s] * sizeof(double), hipMemcpyDeviceToDevice, st));
+                    SCF_HIP(hipMemcpyAsync(vkref[s], vk[s], npair * sizeof(double), hipMemcpyDeviceToDevice, st));
+                }
+                have_ref = true;
             }
         }
         double e = 0.0;
@@ -567,6 +747,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     int prec_prev = prec_E;
     for (int s = 0; s < nspin; ++s) eps_prev[s] = eps[s];
     int num_iter = 1, n_fp64 = 0, start_diis = ctrl.start_diis_cycle;
+    bool last_pseudo = false;
     int nvec = 0, head = 0;   // DIIS ring: vectors [head, head+nvec) mod nd
     bool shift_pending = false;   // first FP64 build after the switch: shift the stored vectors
     std::vector<double> Bmat((size_t)nd * nd, 0.0);
@@ -705,6 +886,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             if (to64) {
                 prec_low = prec;
                 prec = RJK_PREC_FP64;
+                inc_armed = inc_enabled;
                 if (ctrl.switch_mode == RSCF_SWITCH_SHIFT) {
                     shift_pending = nvec > 0;
                 } else {
@@ -717,11 +899,18 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         }
         prec_E = prec_used;
         if (prec_used == RJK_PREC_FP64) ++n_fp64;
-        // diagonalise the extrapolated Fock matrices, new densities
+        // diagonalise the extrapolated Fock matrices, new densities; low-precision iterations
+        // below pseudo_diag_err rotate the previous orbitals instead
+        // (after an incremental reference every iteration does, so that the occupied orbitals
+        // stay aligned with C_ref)
+        const bool pseudo = have_ref || (prec_used != RJK_PREC_FP64 && num_iter >= start_diis && diis_max > 0.0 &&
+                                         diis_max < ctrl.pseudo_diag_err);
+        last_pseudo = pseudo;
         double dm_err[2] = {0.0, 0.0}, eev = 0.0;
         for (int s = 0; s < nspin; ++s) {
             SCF_HIP(hipMemcpyAsync(Dp[s], D[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
-            SCF_RC(diagonalise(Fx[s], s));
+            if (pseudo) SCF_RC(pseudo_diagonalise(Fx[s], s));
+            else SCF_RC(diagonalise(Fx[s], s));
             SCF_RC(density(s, D[s]));
             hipLaunchKernelGGL(k_sumsq_diff, dim3(kRedBlocks), b256, 0, st, D[s], Dp[s], n2, partial);
             SCF_HIP(hipGetLastError());
@@ -758,6 +947,12 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         ++num_iter;
         if (converged || num_iter > ctrl.max_scf_cycle) break;
     }
+    if (last_pseudo && (mo_coeff || mo_energy || dm_out)) {   // canonical orbitals for the outputs
+        for (int s = 0; s < nspin; ++s) {
+            SCF_RC(diagonalise(Fx[s], s));
+            SCF_RC(density(s, D[s]));
+        }
+    }
     SCF_HIP(hipStreamSynchronize(st));
     if (result) {
         result->energy = E;
@@ -770,6 +965,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         result->diis_seconds = t_diis;
         result->guess_energy = E_guess;
         result->guess_rejected = 0;
+        result->incremental_iterations = n_inc;
     }
     for (int s = 0; s < nspin; ++s) {
         if (mo_coeff) SCF_HIP(hipMemcpy(mo_coeff + (size_t)s * n2, A[s], n2 * sizeof(double), hipMemcpyDeviceToHost));

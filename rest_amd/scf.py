@@ -22,7 +22,8 @@ class Ctrl(C.Structure):
                 ("max_scf_cycle", C.c_int), ("scf_acc_rho", C.c_double), ("scf_acc_eev", C.c_double),
                 ("scf_acc_etot", C.c_double), ("jk_prec", C.c_int), ("switch_diis_err", C.c_double),
                 ("print_level", C.c_int), ("switch_mode", C.c_int), ("jk_prec_start", C.c_int),
-                ("start_switch_err", C.c_double), ("guess_energy_max", C.c_double)]
+                ("start_switch_err", C.c_double), ("guess_energy_max", C.c_double), ("pseudo_diag_err", C.c_double),
+                ("jk_prec_increment", C.c_int)]
 
 
 SWITCH_RESTART, SWITCH_SHIFT = 0, 1
@@ -32,7 +33,7 @@ class Result(C.Structure):
     _fields_ = [("energy", C.c_double), ("converged", C.c_int), ("iterations", C.c_int),
                 ("fp64_iterations", C.c_int), ("wall_seconds", C.c_double), ("jk_seconds", C.c_double),
                 ("diag_seconds", C.c_double), ("diis_seconds", C.c_double), ("guess_energy", C.c_double),
-                ("guess_rejected", C.c_int)]
+                ("guess_rejected", C.c_int), ("incremental_iterations", C.c_int)]
 
 
 def _bind(lib):
@@ -86,6 +87,7 @@ class ScfOutcome:
     log: list = field(default_factory=list)
     guess_energy: float = None      # energy of the guess Fock build
     guess_rejected: bool = False    # the guess energy was not below ctrl.guess_energy_max: no iteration ran
+    incremental_iterations: int = 0  # FP64 iterations built as increments on the reference build
 
 
 def _colmajor(M):
@@ -124,7 +126,7 @@ def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals, c_guess=None) -> Scf
     tr = (lambda a: None if a is None else (a[0].T.copy() if ns == 1 else np.stack([x.T.copy() for x in a])))
     return ScfOutcome(res.energy, bool(res.converged), res.iterations, res.fp64_iterations, res.wall_seconds,
                       res.jk_seconds, res.diag_seconds, res.diis_seconds, eps[0] if ns == 1 else eps, tr(C_), tr(dm),
-                      rows, res.guess_energy, bool(res.guess_rejected))
+                      rows, res.guess_energy, bool(res.guess_rejected), res.incremental_iterations)
 
 
 def rhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nelec: int,

@@ -140,6 +140,73 @@ def test_precision_ladder_and_guess_rejection():
         assert not kept.guess_rejected and kept.converged and abs(kept.energy - o64.energy) <= 1e-9
 
 
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_pseudo_diagonalisation_in_low_precision_iterations(nspin):
+    """Below pseudo_diag_err the low-precision iterations rotate the previous orbitals (occupied-virtual
+    first-order step, Cholesky re-orthonormalisation) instead of diagonalising; the FP64 tail
+    diagonalises exactly, so the converged energy is the FP64 one."""
+    say(f"(H2O)4 ladder with pseudo-diagonalisation below 1e-2, nspin {nspin}")
+    m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
+    S, T, V = m.int1e()
+    H = T + V
+    enuc = m.nuclear_repulsion()
+    ne = m.nelectron
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        run = (lambda c: rs.rhf(r, S, H, enuc, ne, None, c)) if nspin == 1 else \
+              (lambda c: rs.uhf(r, S, H, enuc, ne // 2, ne // 2, None, c))
+        o64 = run(tight())
+        ctrl = tight()
+        ctrl.jk_prec, ctrl.jk_prec_start = jk.PREC_BF16X3, jk.PREC_BF16X2
+        ref = run(ctrl)
+        ctrl.pseudo_diag_err = 1e-2
+        om = run(ctrl)
+    precs = [int(x["prec"]) for x in om.log]
+    diis = [x["diis_max"] for x in om.log]
+    say(f"  E {om.energy:.12f} vs {o64.energy:.12f}; iterations {om.iterations} (eigensolver only: "
+        f"{ref.iterations}); precisions {precs}")
+    assert om.converged and abs(om.energy - o64.energy) <= 1e-9
+    assert precs[-1] == jk.PREC_FP64
+    assert any(d < 1e-2 and p != jk.PREC_FP64 for d, p in zip(diis, precs))   # the pseudo branch ran
+    assert om.iterations <= ref.iterations + 2
+
+
+@pytest.mark.parametrize("nspin,pinc", [(1, jk.PREC_BF16X2), (1, jk.PREC_BF16X3), (2, jk.PREC_BF16X2),
+                                        (1, jk.PREC_FP32)])
+def test_incremental_fp64_builds_after_the_switch(nspin, pinc):
+    """After the switch to FP64 the first FP64 build is the reference and later builds add J/K of
+    D − D_ref at a low precision; the converged energy stays the FP64 one and the returned orbitals
+    are canonical (an exact diagonalisation of the last Fock matrix)."""
+    say(f"(H2O)4 ladder with incremental FP64 builds at prec {pinc}, nspin {nspin}")
+    m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
+    S, T, V = m.int1e()
+    H = T + V
+    enuc = m.nuclear_repulsion()
+    ne = m.nelectron
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        run = (lambda c: rs.rhf(r, S, H, enuc, ne, None, c)) if nspin == 1 else \
+              (lambda c: rs.uhf(r, S, H, enuc, ne // 2, ne // 2, None, c))
+        o64 = run(tight())
+        ctrl = tight()
+        ctrl.jk_prec, ctrl.jk_prec_start = jk.PREC_BF16X3, jk.PREC_BF16X2
+        ctrl.jk_prec_increment = pinc
+        om = run(ctrl)
+    precs = [int(x["prec"]) for x in om.log]
+    say(f"  E {om.energy:.12f} vs {o64.energy:.12f} (dE {om.energy - o64.energy:.2e}); iterations {om.iterations}, "
+        f"fp64 {om.fp64_iterations}, incremental {om.incremental_iterations}")
+    assert om.converged and abs(om.energy - o64.energy) <= 1e-9
+    assert om.incremental_iterations >= 1 and om.incremental_iterations == om.fp64_iterations - 1
+    assert precs[-1] == jk.PREC_FP64
+    # canonical outputs: C diagonalises F(D_out) and ε matches the FP64 run's
+    cs = [om.mo_coeff] if nspin == 1 else list(om.mo_coeff)
+    es = [om.mo_energy] if nspin == 1 else list(om.mo_energy)
+    e64 = [o64.mo_energy] if nspin == 1 else list(o64.mo_energy)
+    for c, e, e_ref in zip(cs, es, e64):
+        assert np.allclose(c.T @ S @ c, np.eye(len(e)), atol=1e-10)
+        assert np.max(np.abs(e - e_ref)) < 1e-5
+
+
 def test_mixed_uhf_shift_matches_fp64():
     say("H2O+ doublet UHF: BF16X3 -> FP64 with the shifted DIIS history vs plain FP64")
     m = ri.Molecule.water_cluster(wl.water_cluster(3, 1))
