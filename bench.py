@@ -59,8 +59,12 @@ def parse():
                          "bf16x3 / bf16x2: K on three / two BF16 planes of B (6 / 3 MFMA products), J on the FP32 copy")
     ap.add_argument("--mixed-prec", default="bf16x3", choices=["fp32", "bf16x3"],
                     help="scf mode: J/K precision of the mixed SCF before its switch to FP64")
-    ap.add_argument("--pseudo-diag-err", type=float, default=0.0,
+    ap.add_argument("--pseudo-diag-err", type=float, default=1e-2,
                     help="scf mode: low-precision iterations with max|e| below this pseudo-diagonalise (0: never)")
+    ap.add_argument("--increment-prec", default="bf16x2", choices=["none", "fp32", "bf16x2", "bf16x3"],
+                    help="scf mode: precision of the J/K increments on the FP64 reference build after the switch")
+    ap.add_argument("--guard", default="error", choices=["error", "energy"],
+                    help="scf mode: learned-guess acceptance (guess-build commutator, or energy against SAD's E(D1))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -189,12 +193,14 @@ def main_scf(args):
         return out, {"guess": "sad", "e_guess": e0}
 
     def mixed():
-        ctrl = rs.default_ctrl(jk_prec=mixed_prec)
-        if args.mixed_prec.startswith("bf16"):
-            ctrl.jk_prec_start = jk.PREC_BF16X2        # BF16X2 until max|e| < 1e-3
-        ctrl.pseudo_diag_err = args.pseudo_diag_err
-        g = lg.learned_scf(net, mol, r, S, H, ctrl)
-        return g.outcome, {"guess": g.kind, "e_learned": g.e_learned, "e_sad": g.e_sad, "seconds": g.seconds,
+        ctrl = rs.mixed_ctrl(jk_prec=mixed_prec, pseudo_diag_err=args.pseudo_diag_err,
+                             jk_prec_increment=-1 if args.increment_prec == "none" else jk.PREC_NAMES[args.increment_prec])
+        if not args.mixed_prec.startswith("bf16"):
+            ctrl.jk_prec_start = -1                    # FP32 throughout the low-precision phase
+        g = lg.learned_scf(net, mol, r, S, H, ctrl, guard=args.guard)
+        return g.outcome, {"guess": g.kind, "guard": args.guard, "e_learned": g.e_learned, "e_sad": g.e_sad,
+                           "err_learned": g.err_learned, "seconds": g.seconds,
+                           "incremental_iterations": g.outcome.incremental_iterations,
                            "precisions": [int(x["prec"]) for x in g.outcome.log],
                            "diag_ms": [round(float(x["diag_ms"]), 2) for x in g.outcome.log]}
 
@@ -250,13 +256,16 @@ def main_scf(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 (plain); " + ("f32" if args.mixed_prec == "fp32" else "split-bf16 K (2, then 3 planes) + f32 J")
-                 + " -> f64 J/K, f64 elsewhere (mixed)",
+                 + " -> one f64 J/K reference build"
+                 + ("" if args.increment_prec == "none" else f" + {args.increment_prec} J/K increments on it")
+                 + ", f64 elsewhere (mixed)",
         "data": "synthetic (seeded water cluster geometry; integrals from this build's engine)",
         "config": {"workload": f"(H2O){n} seed {seed}, def2-SVP / def2-SVP-JKFIT, nbf={mol.nbf} naux={mol.naux} "
                                f"nocc={mol.nelectron // 2}; RI-HF to |dE| <= 1e-8 Eh, ||dD||/nbf <= 1e-5, "
                                "||d eps|| <= 1e-5 (REST defaults); FP64+FP32"
                                + ("+BF16-plane" if store & jk.STORE_BF16 else "") + " B resident in HBM",
-                   "mixed_prec": args.mixed_prec,
+                   "mixed_prec": args.mixed_prec, "increment_prec": args.increment_prec,
+                   "pseudo_diag_err": args.pseudo_diag_err, "guard": args.guard,
                    "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nelectron // 2},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -161,6 +161,11 @@ int rjk_vk_upper_with_rimatr_use_dm_only_sync(rjk_ctx* ctx, const double* dm,
 int rjk_vk_upper_with_ri_v_use_dm_only_sync(rjk_ctx* ctx, const double* dm,
                                             size_t spin_channel, double scaling_factor,
                                             int prec, double* vk_upper);
+/* Device form of the density-only route: dm_dev (spin_channel × nbf × nbf, column-major) and
+ * vk_dev (spin_channel × npair) in HBM.  Each D is symmetrised, (D + Dᵀ)/2, instead of being
+ * checked; non-finite eigenvalues give RJK_ERR_ARG. */
+int rjk_vk_upper_dm_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel,
+                        double scaling_factor, int prec, double* vk_dev);
 
 /* ---- measurement -------------------------------------------------------- */
 

@@ -57,6 +57,9 @@ typedef struct rscf_ctrl {
                               * _BF16X2 or _BF16X3); from the reference on every iteration
                               * pseudo-diagonalises, and the outputs come from one exact
                               * diagonalisation at the end */
+    double guess_err_max;    /* return before the first iteration when the guess Fock build's
+                              * max|F D S − S D F| >= this (result.guess_rejected = 1); HUGE_VAL
+                              * (default): never */
 } rscf_ctrl;
 
 /* What DIIS keeps at the precision switch.  RESTART: the stored low-precision
@@ -78,8 +81,10 @@ typedef struct rscf_result {
     double wall_seconds;     /* whole driver, guess Fock build included    */
     double jk_seconds, diag_seconds, diis_seconds;
     double guess_energy;     /* energy of the guess Fock build (the given density or orbitals) */
-    int guess_rejected;      /* 1: guess_energy >= ctrl.guess_energy_max and no iteration ran */
+    int guess_rejected;      /* 1: guess_energy >= ctrl.guess_energy_max or guess_error >=
+                              * ctrl.guess_err_max, and no iteration ran */
     int incremental_iterations; /* FP64 iterations whose J/K were increments on the reference build */
+    double guess_error;      /* max|F D S − S D F| of the guess build */
 } rscf_result;
 
 void rscf_default_ctrl(rscf_ctrl* ctrl);

@@ -575,55 +575,34 @@ int solver_handle(rjk_ctx* ctx, rocblas_handle* h) {
     return RJK_OK;
 }
 
-int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scaling_factor, int prec,
-               double* vk_upper) {
-    if (!ctx) return RJK_ERR_ARG;
-    if (!dm || !vk_upper) return fail(ctx, RJK_ERR_ARG, "vk: NULL host pointer");
-    const void* b = nullptr;
-    int rc = check_common(ctx, spin_channel, prec, true, &b);
-    if (rc) return rc;
+// Density-only K on device memory: each spin's D (dm_dev, column-major) is symmetrised,
+// eigen-decomposed (rocSOLVER dsyevd), and the eigenpairs with |λ| > kEigRelCut·max|λ| go
+// through the occupied route with signed weights λ; K_s = scaling_factor · upper(Σ_P B_P D_s B_P)
+// lands in vk_dev (spin_channel × npair).
+int vk_dm_device(rjk_ctx* ctx, const void* b, const double* dm_dev, size_t spin_channel, double scaling_factor,
+                 int prec, double* vk_dev) {
     const size_t nbf = ctx->nbf, n2 = nbf * nbf;
     const int ns = (int)spin_channel;
-    rc = check_host_finite(ctx, dm, spin_channel * n2, "dm");
-    if (rc) return rc;
-    for (int s = 0; s < ns; ++s) {
-        const double* d = dm + s * n2;
-        double amax = 0, asym = 0;
-        for (size_t j = 0; j < nbf; ++j)
-            for (size_t i = 0; i < nbf; ++i) {
-                amax = std::max(amax, std::fabs(d[i + j * nbf]));
-                asym = std::max(asym, std::fabs(d[i + j * nbf] - d[j + i * nbf]));
-            }
-        if (asym > kAsymTol * std::max(amax, 1e-300))
-            return fail(ctx, RJK_ERR_ARG, "vk (density-only): dm is not symmetric (max|D-D^T| = " +
-                                              std::to_string(asym) + ")");
-    }
     RJK_HIP(ctx, hipSetDevice(ctx->device));
     rocblas_handle h;
-    rc = solver_handle(ctx, &h);
+    int rc = solver_handle(ctx, &h);
     if (rc) return rc;
     hipStream_t st = ctx->stream;
-    // eigen-decompose every spin's D (device), select the non-negligible eigenpairs
     DevBuf io;
-    const size_t out_n = spin_channel * ctx->npair;
-    size_t io_n = spin_channel * n2 /*dm*/ + spin_channel * n2 /*vecs*/ + spin_channel * nbf /*lam*/ + nbf /*E*/ +
-                  spin_channel * nbf /*sel lam*/ + out_n;
+    size_t io_n = spin_channel * n2 /*vecs*/ + spin_channel * nbf /*lam*/ + nbf /*E*/ + spin_channel * nbf /*sel lam*/;
     RJK_HIP(ctx, hipMalloc(&io.p, io_n * sizeof(double) + 2 * spin_channel * nbf * sizeof(int) + 256));
-    double* d_dm = (double*)io.p;
-    double* d_vec = d_dm + spin_channel * n2;
+    double* d_vec = (double*)io.p;
     double* d_lam = d_vec + spin_channel * n2;
     double* d_E = d_lam + spin_channel * nbf;
     double* d_sel = d_E + nbf;
-    double* d_out = d_sel + spin_channel * nbf;
-    int* d_idx = (int*)(d_out + out_n);
+    int* d_idx = (int*)(d_sel + spin_channel * nbf);
     int* d_info = d_idx + spin_channel * nbf;
-    RJK_HIP(ctx, hipMemcpyAsync(d_dm, dm, spin_channel * n2 * sizeof(double), hipMemcpyHostToDevice, st));
     std::vector<double> lam(nbf);
     std::vector<int> sel_idx[2];
     std::vector<double> sel_lam[2];
     size_t ncol[2] = {0, 0};
     for (int s = 0; s < ns; ++s) {
-        RJK_HIP(ctx, launch_symmetrize(st, d_dm + s * n2, nbf, d_vec + s * n2));
+        RJK_HIP(ctx, launch_symmetrize(st, dm_dev + s * n2, nbf, d_vec + s * n2));
         rocblas_status rs = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)nbf,
                                              d_vec + s * n2, (rocblas_int)nbf, d_lam + s * nbf, d_E, d_info);
         if (rs != rocblas_status_success) return fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsyevd failed");
@@ -633,7 +612,10 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
         RJK_HIP(ctx, hipStreamSynchronize(st));
         if (info != 0) return fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsyevd info = " + std::to_string(info));
         double lmax = 0;
-        for (double v : lam) lmax = std::max(lmax, std::fabs(v));
+        for (double v : lam) {
+            if (!std::isfinite(v)) return fail(ctx, RJK_ERR_ARG, "vk (density-only): dm has non-finite entries");
+            lmax = std::max(lmax, std::fabs(v));
+        }
         for (size_t q = 0; q < nbf; ++q)
             if (std::fabs(lam[q]) > kEigRelCut * lmax) {
                 sel_idx[s].push_back((int)q);
@@ -663,7 +645,45 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
             RJK_HIP(ctx, launch_prep_cocc_idx(st, prec, d_vec + s * n2, nbf, d_idx + s * nbf, d_sel + s * nbf,
                                               ncol[s], kp.col_off[s], kp, cocc, sign));
     }
-    rc = k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, d_out);
+    rc = k_run(ctx, kp, ns, prec, b, cocc, sign, kpart, m, scaling_factor, vk_dev);
+    if (rc) return rc;
+    // sel_idx / sel_lam are host sources of the async copies above
+    RJK_HIP(ctx, hipStreamSynchronize(st));
+    return RJK_OK;
+}
+
+int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scaling_factor, int prec,
+               double* vk_upper) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!dm || !vk_upper) return fail(ctx, RJK_ERR_ARG, "vk: NULL host pointer");
+    const void* b = nullptr;
+    int rc = check_common(ctx, spin_channel, prec, true, &b);
+    if (rc) return rc;
+    const size_t nbf = ctx->nbf, n2 = nbf * nbf;
+    const int ns = (int)spin_channel;
+    rc = check_host_finite(ctx, dm, spin_channel * n2, "dm");
+    if (rc) return rc;
+    for (int s = 0; s < ns; ++s) {
+        const double* d = dm + s * n2;
+        double amax = 0, asym = 0;
+        for (size_t j = 0; j < nbf; ++j)
+            for (size_t i = 0; i < nbf; ++i) {
+                amax = std::max(amax, std::fabs(d[i + j * nbf]));
+                asym = std::max(asym, std::fabs(d[i + j * nbf] - d[j + i * nbf]));
+            }
+        if (asym > kAsymTol * std::max(amax, 1e-300))
+            return fail(ctx, RJK_ERR_ARG, "vk (density-only): dm is not symmetric (max|D-D^T| = " +
+                                              std::to_string(asym) + ")");
+    }
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DevBuf io;
+    const size_t out_n = spin_channel * ctx->npair;
+    RJK_HIP(ctx, hipMalloc(&io.p, (spin_channel * n2 + out_n) * sizeof(double)));
+    double* d_dm = (double*)io.p;
+    double* d_out = d_dm + spin_channel * n2;
+    RJK_HIP(ctx, hipMemcpyAsync(d_dm, dm, spin_channel * n2 * sizeof(double), hipMemcpyHostToDevice, st));
+    rc = vk_dm_device(ctx, b, d_dm, spin_channel, scaling_factor, prec, d_out);
     if (rc) return rc;
     RJK_HIP(ctx, hipMemcpyAsync(vk_upper, d_out, out_n * sizeof(double), hipMemcpyDeviceToHost, st));
     RJK_HIP(ctx, hipStreamSynchronize(st));
@@ -671,6 +691,16 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
 }
 
 }  // namespace
+
+extern "C" int rjk_vk_upper_dm_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel, double scaling_factor,
+                                  int prec, double* vk_dev) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!dm_dev || !vk_dev) return fail(ctx, RJK_ERR_ARG, "vk: NULL device pointer");
+    const void* b = nullptr;
+    int rc = check_common(ctx, spin_channel, prec, true, &b);
+    if (rc) return rc;
+    return vk_dm_device(ctx, b, dm_dev, spin_channel, scaling_factor, prec, vk_dev);
+}
 
 extern "C" int rjk_vk_upper_with_rimatr_use_dm_only_sync(rjk_ctx* ctx, const double* dm, size_t spin_channel,
                                                          double scaling_factor, int prec, double* vk_upper) {

@@ -323,6 +323,7 @@ extern "C" void rscf_default_ctrl(rscf_ctrl* c) {
     c->guess_energy_max = HUGE_VAL;
     c->pseudo_diag_err = 0.0;
     c->jk_prec_increment = -1;
+    c->guess_err_max = HUGE_VAL;
 }
 
 namespace {
@@ -640,12 +641,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                     const size_t homo[2] = {nocc[s] - 1, 0};
                     SCF_RC(rjk_vk_upper_occ_dev(ctx, A[s], n, homo, occd[s], 1, sk, prec, vk[s]));
                 } else {
-                    std::vector<double> dh(n2), vkh(npair);
-                    SCF_HIP(hipMemcpyAsync(dh.data(), Dsrc[s], n2 * sizeof(double), hipMemcpyDeviceToHost, st));
-                    SCF_HIP(hipStreamSynchronize(st));
-                    SCF_RC(rjk_vk_upper_with_rimatr_use_dm_only_sync(ctx, dh.data(), 1, sk, prec, vkh.data()));
-                    SCF_HIP(hipMemcpyAsync(vk[s], vkh.data(), npair * sizeof(double), hipMemcpyHostToDevice, st));
-                    SCF_HIP(hipStreamSynchronize(st));   // vkh is a host temporary
+                    SCF_RC(rjk_vk_upper_dm_dev(ctx, Dsrc[s], 1, sk, prec, vk[s]));
                 }
             }
             if (inc_route) {   // this FP64 build is the reference of the increments that follow
@@ -697,12 +693,31 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     double E = 0.0;
     SCF_RC(fock(Dp, guess_from_orbitals, prec, &E));
     const double E_guess = E;
-    if (!energy_only && !guess_only && E >= ctrl.guess_energy_max) {   // guess not good enough: stop here
+    // the guess's own self-consistency error, max|F D S − S D F| over the spins
+    double guess_err = 0.0;
+    if (!energy_only && !guess_only) {
+        for (int s = 0; s < nspin; ++s) {
+            SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                                   (rocblas_int)n, &one, F[s], (rocblas_int)n, Dp[s], (rocblas_int)n, &zero, T1,
+                                   (rocblas_int)n));
+            SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                                   (rocblas_int)n, &one, T1, (rocblas_int)n, S, (rocblas_int)n, &zero, T2,
+                                   (rocblas_int)n));
+            hipLaunchKernelGGL(k_commutator, dim3(kRedBlocks), b256, 0, st, T2, n, Bw, partial);
+            SCF_HIP(hipGetLastError());
+            double m = 0.0;
+            SCF_RC(reduce_max(&m));
+            guess_err = std::max(guess_err, m);
+        }
+    }
+    if (!energy_only && !guess_only && (E >= ctrl.guess_energy_max || guess_err >= ctrl.guess_err_max)) {
+        // guess not good enough: stop here
         SCF_HIP(hipStreamSynchronize(st));
         if (result) {
             *result = rscf_result{};
             result->energy = E;
             result->guess_energy = E;
+            result->guess_error = guess_err;
             result->guess_rejected = 1;
             result->wall_seconds = now() - t_start;
             result->jk_seconds = t_jk;
@@ -964,6 +979,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         result->diag_seconds = t_diag;
         result->diis_seconds = t_diis;
         result->guess_energy = E_guess;
+        result->guess_error = guess_err;
         result->guess_rejected = 0;
         result->incremental_iterations = n_inc;
     }

@@ -45,7 +45,7 @@ EXPORTS = [
     "rjk_set_rimatr_dev", "rjk_set_ri3fn", "rjk_fill_rimatr_synthetic", "rjk_rimatr_dev", "rjk_get_rimatr",
     "rjk_vj_upper_with_rimatr_v_sync", "rjk_vj_upper_with_ri_v_sync", "rjk_vj_upper_dev",
     "rjk_vk_upper_with_ri_v_sync", "rjk_vk_upper_occ_dev", "rjk_vk_upper_with_rimatr_use_dm_only_sync",
-    "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_profile_enable", "rjk_profile_read", "rjk_profile_read_detail", "rjk_copy_dev",
+    "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_vk_upper_dm_dev", "rjk_profile_enable", "rjk_profile_read", "rjk_profile_read_detail", "rjk_copy_dev",
     # include/rest_ri.h
     "rsi_basis_create", "rsi_basis_destroy", "rsi_basis_nbf", "rsi_last_error", "rsi_int1e", "rsi_int2c2e",
     "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr", "rsi_cart2sph",
@@ -112,6 +112,7 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rjk_vk_upper_occ_dev": (i, [vp, vp, sz, vp, vp, sz, d, i, vp]),
         "rjk_vk_upper_with_rimatr_use_dm_only_sync": (i, [vp, vp, sz, d, i, vp]),
         "rjk_vk_upper_with_ri_v_use_dm_only_sync": (i, [vp, vp, sz, d, i, vp]),
+        "rjk_vk_upper_dm_dev": (i, [vp, vp, sz, d, i, vp]),
         "rjk_profile_enable": (i, [vp, i]),
         "rjk_profile_read": (i, [vp, pd, C.POINTER(C.c_uint64)]),
         "rjk_profile_read_detail": (i, [vp, pd, C.POINTER(C.c_uint64), pd]),
@@ -301,6 +302,11 @@ class DeviceRI:
         self._check(self.lib.rjk_vk_upper_occ_dev(self.h, C.c_void_p(eigv_ptr), num_state, C.cast(hm, C.c_void_p),
                                                   C.c_void_p(occ_ptr), spin_channel, float(scaling_factor), prec,
                                                   C.c_void_p(out_ptr)))
+
+    def vk_dm_dev(self, dm_ptr: int, spin_channel: int, scaling_factor: float, out_ptr: int,
+                  prec: int = PREC_FP64):
+        self._check(self.lib.rjk_vk_upper_dm_dev(self.h, C.c_void_p(dm_ptr), spin_channel, float(scaling_factor),
+                                                 prec, C.c_void_p(out_ptr)))
 
     # -- measurement -------------------------------------------------------------
     def profile_enable(self, on: bool = True):

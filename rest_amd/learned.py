@@ -630,26 +630,42 @@ def learned_guess(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, prec: in
     return LearnedGuess(c1_l if ok else c1_s, "learned" if ok else "sad", e_l, e_s, ne, secs)
 
 
+# Acceptance bar of the "error" guard: max|F D S - S D F| of the purified prediction's guess
+# Fock build (BF16X2 J/K).  Measured on (H2O)8, 16, 32 and three (H2O)64: the packaged model's
+# predictions give 0.037-0.064, SAD's first orbital density (REST's guess step) 0.22-0.26 and the
+# SAD density itself about 1.9; a prediction at 0.15 or above is no clear gain over SAD's step.
+GUESS_ERR_MAX = 0.15
+
+
 @dataclass
 class LearnedScf:
     outcome: "rs.ScfOutcome"    # the SCF that was kept
     kind: str                   # "learned" or "sad"
     e_learned: float            # E(D) of the purified prediction (the SCF's guess Fock build)
-    e_sad: float                # E(D1) of SAD's first orbital density
+    e_sad: float                # E(D1) of SAD's first orbital density ("energy" guard; None otherwise)
     electrons: float            # 2 tr(PS) of the purified prediction
     seconds: dict = field(default_factory=dict)
+    err_learned: float = None   # max|F D S - S D F| of the prediction's guess build
 
 
 def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
-                mcweeny_iterations: int = 6) -> LearnedScf:
-    """RHF from the learned guess, with the acceptance rule of learned_guess folded into the SCF.
+                mcweeny_iterations: int = 6, guard: str = "error",
+                guess_err_max: float = GUESS_ERR_MAX) -> LearnedScf:
+    """RHF from the learned guess, with the acceptance rule folded into the SCF.
 
-    SAD's E(D1) is computed first (REST's guess step from the SAD density plus the energy of the
-    new orbitals' density).  The SCF then starts from the purified prediction: its guess Fock
-    build is F(D_pred), whose energy decides: below E(D1_SAD) the SCF continues from there
-    (REST's guess-step sequence, without a separate guess-step build); otherwise it stops at once
-    and the SCF runs from SAD's first orbitals instead.  Both energies are evaluated at the SCF's
-    starting J/K precision."""
+    The SCF starts from the purified prediction; its guess Fock build F(D_pred) decides, before
+    the first iteration, whether the prediction is kept (REST's guess-step sequence continues
+    from it without a separate guess-step build) or the SCF stops at once and runs from SAD
+    instead (REST's guess step from the SAD density).
+      guard "error" (default): kept when the guess build's self-consistency error
+        max|F D S - S D F| is below guess_err_max; the test costs two GEMMs.
+      guard "energy": SAD's E(D1) is computed first (REST's guess step from the SAD density plus
+        the energy of the new orbitals' density, at the SCF's starting precision); kept when
+        E(D_pred) is below it.  Both are energies of idempotent densities with the molecule's
+        electron count, so both bound the converged energy from above and compare fairly.
+    A prediction that is not finite or misses the electron count by 0.5 or more is never kept."""
+    if guard not in ("error", "energy"):
+        raise ValueError(f"learned_scf: guard must be 'error' or 'energy', not {guard!r}")
     ctrl = ctrl or rs.default_ctrl()
     prec0 = ctrl.jk_prec_start if ctrl.jk_prec_start >= 0 else ctrl.jk_prec
     t0 = time.perf_counter()
@@ -662,23 +678,31 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
     d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
     t3 = time.perf_counter()
     enuc = mol.nuclear_repulsion()
-    _, _, c1_s, _, e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True)
+    e_s, c1_s = None, None
+    if guard == "energy":
+        _, _, c1_s, _, e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True)
     t4 = time.perf_counter()
     ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
-    out, e_l = None, float("inf")
+    out, e_l, err_l = None, float("inf"), None
     if ok:
         c2 = rs.Ctrl.from_buffer_copy(ctrl)
-        c2.guess_energy_max = e_s
+        if guard == "energy":
+            c2.guess_energy_max = e_s
+        else:
+            c2.guess_err_max = guess_err_max
         out = rs.rhf(r, S, H, enuc, mol.nelectron, d_pur, c2)
-        e_l = out.guess_energy
+        e_l, err_l = out.guess_energy, out.guess_error
         if out.guess_rejected or not np.isfinite(e_l):
             out = None
     kind = "learned" if out is not None else "sad"
     if out is None:
-        out = rs.rhf(r, S, H, enuc, mol.nelectron, None, ctrl, c_guess=c1_s)
+        if c1_s is not None:
+            out = rs.rhf(r, S, H, enuc, mol.nelectron, None, ctrl, c_guess=c1_s)
+        else:
+            out = rs.rhf(r, S, H, enuc, mol.nelectron, d_sad, ctrl)
     t5 = time.perf_counter()
     secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "scf": t5 - t4}
-    return LearnedScf(out, kind, e_l, e_s, ne, secs)
+    return LearnedScf(out, kind, e_l, e_s, ne, secs, err_l)
 
 
 # ---------------------------------------------------------------------------

@@ -23,7 +23,7 @@ class Ctrl(C.Structure):
                 ("scf_acc_etot", C.c_double), ("jk_prec", C.c_int), ("switch_diis_err", C.c_double),
                 ("print_level", C.c_int), ("switch_mode", C.c_int), ("jk_prec_start", C.c_int),
                 ("start_switch_err", C.c_double), ("guess_energy_max", C.c_double), ("pseudo_diag_err", C.c_double),
-                ("jk_prec_increment", C.c_int)]
+                ("jk_prec_increment", C.c_int), ("guess_err_max", C.c_double)]
 
 
 SWITCH_RESTART, SWITCH_SHIFT = 0, 1
@@ -33,7 +33,8 @@ class Result(C.Structure):
     _fields_ = [("energy", C.c_double), ("converged", C.c_int), ("iterations", C.c_int),
                 ("fp64_iterations", C.c_int), ("wall_seconds", C.c_double), ("jk_seconds", C.c_double),
                 ("diag_seconds", C.c_double), ("diis_seconds", C.c_double), ("guess_energy", C.c_double),
-                ("guess_rejected", C.c_int), ("incremental_iterations", C.c_int)]
+                ("guess_rejected", C.c_int), ("incremental_iterations", C.c_int),
+                ("guess_error", C.c_double)]
 
 
 def _bind(lib):
@@ -71,6 +72,17 @@ def default_ctrl(**kw) -> Ctrl:
     return c
 
 
+def mixed_ctrl(**kw) -> Ctrl:
+    """The mixed-precision ladder: BF16X2 J/K until max|e| < 1e-3, BF16X3 until < 1e-5, then one
+    FP64 reference build and FP64-referenced increments at BF16X2; low-precision iterations below
+    max|e| = 1e-2 and every iteration from the reference on pseudo-diagonalise."""
+    c = default_ctrl(jk_prec=jk.PREC_BF16X3, jk_prec_start=jk.PREC_BF16X2, pseudo_diag_err=1e-2,
+                     jk_prec_increment=jk.PREC_BF16X2)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
 @dataclass
 class ScfOutcome:
     energy: float
@@ -86,12 +98,25 @@ class ScfOutcome:
     dm: np.ndarray = None
     log: list = field(default_factory=list)
     guess_energy: float = None      # energy of the guess Fock build
-    guess_rejected: bool = False    # the guess energy was not below ctrl.guess_energy_max: no iteration ran
+    guess_rejected: bool = False    # the guess failed ctrl.guess_energy_max or ctrl.guess_err_max: no iteration ran
     incremental_iterations: int = 0  # FP64 iterations built as increments on the reference build
+    guess_error: float = None       # max|F D S - S D F| of the guess build
 
 
 def _colmajor(M):
     return np.ascontiguousarray(np.asarray(M, dtype=np.float64).T)
+
+
+def _sym(M):
+    """A symmetric matrix (overlap, core Hamiltonian, density): its row-major memory is already
+    its column-major layout, so no transposing copy is made."""
+    return np.ascontiguousarray(M, dtype=np.float64)
+
+
+def _from_colmajor(a, symmetric: bool):
+    """A column-major n x n output as a row-major-indexed array: the buffer itself for a symmetric
+    matrix, else its transposed view (no copy either way)."""
+    return a if symmetric else a.T
 
 
 def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals, c_guess=None) -> ScfOutcome:
@@ -99,11 +124,11 @@ def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals, c_guess=None) -> Scf
     n = ri.nbf
     ns = len(nel)
     ctrl = ctrl or default_ctrl()
-    Sf, Hf = _colmajor(S), _colmajor(H)
+    Sf, Hf = _sym(S), _sym(H)
     dg = None
     if d_guess is not None:
         dgs = [d_guess] if ns == 1 else list(d_guess)
-        dg = np.ascontiguousarray(np.stack([_colmajor(d) for d in dgs]))
+        dg = np.ascontiguousarray(np.stack([_sym(d) for d in dgs]))
     res = Result()
     C_ = np.empty((ns, n, n)) if want_orbitals else None
     eps = np.empty((ns, n))
@@ -123,10 +148,12 @@ def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals, c_guess=None) -> Scf
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
     rows = [dict(zip(LOG_FIELDS, r)) for r in log[:res.iterations]]
-    tr = (lambda a: None if a is None else (a[0].T.copy() if ns == 1 else np.stack([x.T.copy() for x in a])))
+    tr = (lambda a, sym: None if a is None else
+          (_from_colmajor(a[0], sym) if ns == 1 else (a if sym else a.transpose(0, 2, 1))))
     return ScfOutcome(res.energy, bool(res.converged), res.iterations, res.fp64_iterations, res.wall_seconds,
-                      res.jk_seconds, res.diag_seconds, res.diis_seconds, eps[0] if ns == 1 else eps, tr(C_), tr(dm),
-                      rows, res.guess_energy, bool(res.guess_rejected), res.incremental_iterations)
+                      res.jk_seconds, res.diag_seconds, res.diis_seconds, eps[0] if ns == 1 else eps, tr(C_, False), tr(dm, True),
+                      rows, res.guess_energy, bool(res.guess_rejected), res.incremental_iterations,
+                      res.guess_error)
 
 
 def rhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nelec: int,
@@ -150,21 +177,20 @@ def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PRE
     n = ri.nbf
     ns = len(nel)
     d0s = [d0] if ns == 1 else list(d0)
-    d0f = np.ascontiguousarray(np.stack([_colmajor(d) for d in d0s]))
+    d0f = np.ascontiguousarray(np.stack([_sym(d) for d in d0s]))
     d1 = np.empty((ns, n, n))
     C_ = np.empty((ns, n, n))
     eps = np.empty((ns, n))
     nl = (C.c_size_t * 2)(*(list(nel) + [0] * (2 - ns)))
     e = C.c_double(0.0)
     e1 = C.c_double(0.0)
-    Sf, Hf = _colmajor(S), _colmajor(H)      # kept alive across the call
+    Sf, Hf = _sym(S), _sym(H)      # kept alive across the call
     rc = lib.rscf_guess_step(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, d0f.ctypes.data, int(prec),
                              C.byref(e), d1.ctypes.data, C_.ctypes.data, eps.ctypes.data,
                              C.byref(e1) if want_e1 else None)
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
-    d1 = np.stack([x.T.copy() for x in d1])
-    C_ = np.stack([x.T.copy() for x in C_])
+    C_ = C_.transpose(0, 2, 1)     # column-major outputs; d1 is symmetric
     out = (e.value, (d1[0] if ns == 1 else d1), (C_[0] if ns == 1 else C_), (eps[0] if ns == 1 else eps))
     return out + (e1.value,) if want_e1 else out
 
@@ -174,9 +200,9 @@ def energy(ri: "jk.DeviceRI", S, H, enuc: float, nel, dm, prec: int = jk.PREC_FP
     lib = _bind(jk.load_library())
     ns = len(nel)
     dms = [dm] if ns == 1 else list(dm)
-    df = np.ascontiguousarray(np.stack([_colmajor(d) for d in dms]))
+    df = np.ascontiguousarray(np.stack([_sym(d) for d in dms]))
     nl = (C.c_size_t * 2)(*(list(nel) + [0] * (2 - ns)))
-    Sf, Hf = _colmajor(S), _colmajor(H)
+    Sf, Hf = _sym(S), _sym(H)
     e = C.c_double(0.0)
     rc = lib.rscf_energy(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, df.ctypes.data, int(prec),
                          C.byref(e))
@@ -190,11 +216,11 @@ def mcweeny(ri: "jk.DeviceRI", S, D, iterations: int = 3):
     lib = _bind(jk.load_library())
     out = np.empty_like(np.asarray(D, dtype=np.float64))
     ne = C.c_double(0.0)
-    Sf, Df = _colmajor(S), _colmajor(D)      # kept alive across the call
+    Sf, Df = _sym(S), _sym(D)      # kept alive across the call
     rc = lib.rscf_mcweeny(ri.h, Sf.ctypes.data, Df.ctypes.data, int(iterations), out.ctypes.data, C.byref(ne))
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
-    return out.T.copy(), ne.value
+    return _from_colmajor(out, True), ne.value
 
 
 def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctrl] = None,

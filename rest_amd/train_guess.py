@@ -15,9 +15,11 @@ the GPU mostly idle in a single process); every worker stops at the
 generation deadline and the training uses the clusters done by then.
 Held-out test: (H2O)n clusters from workload.water_cluster with seeds not used
 in training (--test-n, --test-seeds); for each, the FP64 SCF from SAD and the
-FP64 SCF from the learned guess (learned.learned_scf: kept only when its
-energy is below that of SAD's first orbital density), with iterations, guess
-energies and final energies.
+SCF from the learned guess (learned.learned_scf with the --guard acceptance
+rule) at FP64 and, with --mixed-eval or --mixed-only, on the mixed-precision
+ladder (scf.mixed_ctrl), with iterations, guess energies and errors of both
+guesses (the learned prediction's and SAD's first orbital density's) and final
+energies.
 """
 from __future__ import annotations
 
@@ -38,7 +40,7 @@ def log(msg):
     sys.stdout.flush()
 
 
-def evaluate(net, n: int, seeds, prec_mixed: bool = False) -> list:
+def evaluate(net, n: int, seeds, prec_mixed: bool = False, guard: str = "error") -> list:
     from . import guess as gs
     from . import jk, ri
     from . import learned as lg
@@ -57,17 +59,18 @@ def evaluate(net, n: int, seeds, prec_mixed: bool = False) -> list:
             e0, _, c1, _ = rs.guess_step(r, S, H, m.nuclear_repulsion(), (m.nelectron,), da + db)
             o_sad = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, None, rs.default_ctrl(), c_guess=c1)
             t1 = time.perf_counter()
-            ctrl = rs.default_ctrl()
-            if prec_mixed:
-                ctrl.jk_prec, ctrl.jk_prec_start = jk.PREC_BF16X3, jk.PREC_BF16X2
-            g = lg.learned_scf(net, m, r, S, H, ctrl)
+            ctrl = rs.mixed_ctrl() if prec_mixed else rs.default_ctrl()
+            g = lg.learned_scf(net, m, r, S, H, ctrl, guard=guard)
             t2 = time.perf_counter()
         o_l = g.outcome
         row = {"n": n, "seed": seed, "sad_iterations": o_sad.iterations, "sad_energy": o_sad.energy,
-               "sad_guess_energy": e0, "sad_seconds": t1 - t0, "learned_kind": g.kind,
-               "learned_guess_energy": g.e_learned, "sad_d1_energy": g.e_sad, "learned_electrons": g.electrons,
+               "sad_guess_energy": e0, "sad_seconds": t1 - t0, "learned_kind": g.kind, "guard": guard,
+               "learned_guess_energy": g.e_learned, "sad_d1_energy": o_sad.guess_energy,
+               "learned_guess_error": g.err_learned, "sad_d1_error": o_sad.guess_error,
+               "learned_electrons": g.electrons,
                "learned_iterations": o_l.iterations, "learned_fp64_iterations": o_l.fp64_iterations,
                "learned_energy": o_l.energy, "learned_seconds": t2 - t1, "dE": o_l.energy - o_sad.energy,
+               "learned_incremental_iterations": o_l.incremental_iterations,
                "converged": bool(o_sad.converged and o_l.converged), "mixed": prec_mixed}
         log(json.dumps(row))
         rows.append(row)
@@ -133,6 +136,8 @@ def main(argv=None):
     ap.add_argument("--test-n", type=int, default=64)
     ap.add_argument("--test-seeds", type=int, nargs="*", default=[2])
     ap.add_argument("--mixed-eval", action="store_true", help="also evaluate the mixed-precision SCF")
+    ap.add_argument("--mixed-only", action="store_true", help="evaluate the mixed-precision SCF only")
+    ap.add_argument("--guard", default="error", choices=["error", "energy"], help="learned-guess acceptance rule")
     ap.add_argument("--report", type=Path, default=None)
     ap.add_argument("--shard", type=int, nargs=2, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--shard-out", type=Path, default=None, help=argparse.SUPPRESS)
@@ -144,12 +149,12 @@ def main(argv=None):
     if a.eval_only:
         from . import learned as lg
         net = lg.Net.load(a.model)
-        rows = evaluate(net, a.test_n, a.test_seeds)
-        if a.mixed_eval:
-            rows += evaluate(net, a.test_n, a.test_seeds, prec_mixed=True)
+        rows = [] if a.mixed_only else evaluate(net, a.test_n, a.test_seeds, guard=a.guard)
+        if a.mixed_eval or a.mixed_only:
+            rows += evaluate(net, a.test_n, a.test_seeds, prec_mixed=True, guard=a.guard)
         if a.report:
-            a.report.write_text(json.dumps({"model": str(a.model), "meta": getattr(net, "meta", {}), "test": rows},
-                                           indent=1))
+            a.report.write_text(json.dumps({"model": str(a.model), "meta": getattr(net, "meta", {}), "guard": a.guard,
+                                            "test": rows}, indent=1))
         return 0
 
     if a.out is None:

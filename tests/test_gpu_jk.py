@@ -173,6 +173,37 @@ def test_density_only_route_indefinite_d():
             assert maxabs(vk, ref) <= TOL
 
 
+def test_density_only_device_route():
+    """rjk_vk_upper_dm_dev: the density-only route on device memory equals the host entry point
+    bit for bit, matches the oracle, and symmetrises an asymmetric D where the host route rejects it."""
+    import torch
+    nbf, naux = 40, 90
+    say(f"density-only device route nbf={nbf} naux={naux}")
+    rng = np.random.default_rng(45)
+    npair = nbf * (nbf + 1) // 2
+    b = rng.standard_normal((npair, naux)) / np.sqrt(naux)
+    d = rng.standard_normal((nbf, nbf))
+    d = 0.5 * (d + d.T)
+    skew = 1e-3 * rng.standard_normal((nbf, nbf))
+    skew = skew - skew.T
+    dev = torch.device("cuda:0")
+    with jk.DeviceRI(nbf, naux) as ri:
+        ri.set_rimatr(b)
+        host = ri.vk_dm([d], 1, 0.5)
+        out = torch.empty((1, npair), dtype=torch.float64, device=dev)
+        for dm, exact in ((d, True), (d + skew, False)):
+            dm_dev = torch.from_numpy(np.ascontiguousarray(dm.T)).to(dev)   # column-major
+            torch.cuda.synchronize()
+            ri.vk_dm_dev(dm_dev.data_ptr(), 1, 0.5, out.data_ptr())
+            ri.synchronize()
+            got = out.cpu().numpy()
+            # the skewed D symmetrises to d up to the rounding of d + skew
+            assert np.array_equal(got, host) if exact else maxabs(got, host) <= 1e-13
+        assert maxabs(host, orc.vk_upper_dm(b, True, [d], 1, 0.5)) <= TOL
+        with pytest.raises(jk.RJKError):
+            ri.vk_dm([d + skew], 1, 0.5)
+
+
 def test_workspace_chunking_matches_single_pass():
     nbf, naux, nocc = 200, 333, 21
     say(f"chunked K nbf={nbf} naux={naux}")

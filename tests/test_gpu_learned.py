@@ -138,17 +138,19 @@ def test_learned_guess_falls_back_to_sad_for_a_useless_network():
     assert out.converged and abs(out.energy - ref.energy) < 1e-9
 
 
-@pytest.mark.parametrize("use_model", [True, False])
-def test_learned_scf_matches_fp64_and_rejects_by_energy(use_model):
-    """learned_scf: SAD's E(D1) first, then the SCF from the purified prediction, kept only when its
-    guess Fock energy is below E(D1_SAD); BF16X2 start, BF16X3, then FP64.  The packaged model
-    (kept or not) and an untrained network (rejected or kept) both end at the FP64 SAD-started energy."""
+@pytest.mark.parametrize("use_model,guard", [(True, "energy"), (False, "energy"), (True, "error")])
+def test_learned_scf_matches_fp64_and_rejects(use_model, guard):
+    """learned_scf: the SCF from the purified prediction, kept by the guard ("energy": its guess Fock
+    energy below SAD's E(D1); "error": its guess build's max|F D S - S D F| below GUESS_ERR_MAX),
+    else from SAD; on the mixed ladder (BF16X2, BF16X3, FP64 reference + increments).  The packaged
+    model (kept) and an untrained network (rejected) both end at the FP64 SAD-started energy, and a
+    forced rejection (a bar no guess meets) does too."""
     from pathlib import Path
     from rest_amd import guess as gs
     from rest_amd import jk, ri
     from rest_amd import scf as rs
     from rest_amd import workload as wl
-    say(f"(H2O)8 learned_scf, {'packaged model' if use_model else 'untrained network'}")
+    say(f"(H2O)8 learned_scf, {'packaged model' if use_model else 'untrained network'}, guard {guard}")
     m = ri.Molecule.water_cluster(wl.water_cluster(5, 8))
     S, T_, V = m.int1e()
     H = T_ + V
@@ -159,13 +161,20 @@ def test_learned_scf_matches_fp64_and_rejects_by_energy(use_model):
         m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
         ref = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, gs.sad_guess(m, r, S, H).d1,
                      rs.default_ctrl(**tight))
-        ctrl = rs.default_ctrl(jk_prec=jk.PREC_BF16X3, jk_prec_start=jk.PREC_BF16X2, **tight)
-        g = lg.learned_scf(net, m, r, S, H, ctrl)
+        ctrl = rs.mixed_ctrl(**tight)
+        g = lg.learned_scf(net, m, r, S, H, ctrl, guard=guard)
+        forced = lg.learned_scf(net, m, r, S, H, ctrl, guard=guard, guess_err_max=1e-12) if guard == "error" else None
     precs = [int(x["prec"]) for x in g.outcome.log]
-    say(f"  kind {g.kind}: E_learned {g.e_learned:.6f} E_sad {g.e_sad:.6f}; SCF {g.outcome.energy:.10f} "
-        f"({g.outcome.iterations} it, precisions {precs}) vs FP64 SAD-started {ref.energy:.10f}")
-    assert (g.kind == "learned") == (g.e_learned < g.e_sad)
+    say(f"  kind {g.kind}: E_learned {g.e_learned:.6f} E_sad {g.e_sad} err {g.err_learned}; SCF "
+        f"{g.outcome.energy:.10f} ({g.outcome.iterations} it, precisions {precs}, incremental "
+        f"{g.outcome.incremental_iterations}) vs FP64 SAD-started {ref.energy:.10f}")
+    if guard == "energy":
+        assert (g.kind == "learned") == (g.e_learned < g.e_sad)
+    else:
+        assert g.e_sad is None and g.kind == "learned" and g.err_learned < lg.GUESS_ERR_MAX
+        assert forced.kind == "sad" and forced.outcome.converged
+        assert abs(forced.outcome.energy - ref.energy) <= 1e-9
     assert g.outcome.converged and abs(g.outcome.energy - ref.energy) <= 1e-9
-    assert precs[-1] == jk.PREC_FP64
+    assert precs[-1] == jk.PREC_FP64 and g.outcome.incremental_iterations >= 1
     if use_model:
         assert g.kind == "learned"
