@@ -230,6 +230,17 @@ def main_scf(args):
     op, ip, om, im = runs[-1]
     t_plain, t_mixed = float(np.median(tp)), float(np.median(tm))
     roofline = roofline_from_detail(detail_prof)
+    # outside the timed region: the same mixed ladder from SAD (REST's guess step inside the driver),
+    # to separate what the learned guess and what the ladder contribute
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    da, db = gs.sad_density(mol)
+    oms = rs.rhf(r, S, H, enuc, mol.nelectron, da + db, rs.mixed_ctrl(
+        jk_prec=mixed_prec, pseudo_diag_err=args.pseudo_diag_err,
+        jk_prec_increment=-1 if args.increment_prec == "none" else jk.PREC_NAMES[args.increment_prec]))
+    torch.cuda.synchronize()
+    mixed_sad = {"scf_s": time.perf_counter() - a, "iterations": oms.iterations, "energy": oms.energy,
+                 "converged": oms.converged, "precisions": [int(x["prec"]) for x in oms.log]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -277,6 +288,7 @@ def main_scf(args):
                               "diag_s": om.diag_seconds, **im},
             "dE_mixed_minus_plain": om.energy - op.energy,
             "all_steps_plain_s": tp, "all_steps_mixed_s": tm,
+            "mixed_from_sad_untimed": mixed_sad,
             "setup_s": {"int1e": t1 - t0, "b_build": t2 - t1, "b_build_parts": binfo, "sad_atoms_and_model": t3 - t2},
             "kernels": {f"{p}:{ph}": {"ms": ms, "launches": k, "work": w} for (p, ph), (ms, k, w) in
                         detail_prof.items() if k},

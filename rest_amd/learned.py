@@ -631,9 +631,10 @@ def learned_guess(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, prec: in
 
 
 # Acceptance bar of the "error" guard: max|F D S - S D F| of the purified prediction's guess
-# Fock build (BF16X2 J/K).  Measured on (H2O)8, 16, 32 and three (H2O)64: the packaged model's
-# predictions give 0.037-0.064, SAD's first orbital density (REST's guess step) 0.22-0.26 and the
-# SAD density itself about 1.9; a prediction at 0.15 or above is no clear gain over SAD's step.
+# Fock build (BF16X2 J/K).  Measured with the packaged model: 0.044-0.095 on 64 held-out (H2O)64
+# and 0.037-0.052 on (H2O)8, 16 and 32, against 0.23-0.27 for SAD's first orbital density (REST's
+# guess step) and about 1.9 for the SAD density itself; a prediction at 0.15 or above is no clear
+# gain over SAD's step.
 GUESS_ERR_MAX = 0.15
 
 

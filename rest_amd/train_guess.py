@@ -77,6 +77,32 @@ def evaluate(net, n: int, seeds, prec_mixed: bool = False, guard: str = "error")
     return rows
 
 
+def summarize(rows: list) -> dict:
+    """Means and extremes of an evaluation's rows (per precision mode)."""
+    out = {}
+    for mode in sorted({bool(r["mixed"]) for r in rows}):
+        rs_ = [r for r in rows if bool(r["mixed"]) == mode]
+        col = lambda k: np.array([r[k] for r in rs_ if r.get(k) is not None], dtype=float)
+        rng = lambda k: [float(col(k).min()), float(col(k).mean()), float(col(k).max())] if len(col(k)) else None
+        out["mixed" if mode else "fp64"] = {
+            "clusters": len(rs_), "kept": sum(r["learned_kind"] == "learned" for r in rs_),
+            "converged": sum(bool(r["converged"]) for r in rs_),
+            "sad_iterations_mean": float(col("sad_iterations").mean()),
+            "learned_iterations_mean": float(col("learned_iterations").mean()),
+            "max_abs_dE": float(np.abs(col("dE")).max()),
+            "sad_seconds_mean": float(col("sad_seconds").mean()),
+            "learned_seconds_mean": float(col("learned_seconds").mean()),
+            "learned_guess_error_min_mean_max": rng("learned_guess_error"),
+            "sad_d1_error_min_mean_max": rng("sad_d1_error"),
+        }
+        gap = np.array([r["learned_guess_energy"] - r["sad_d1_energy"] for r in rs_
+                        if r.get("learned_guess_energy") is not None and r.get("sad_d1_energy") is not None])
+        if len(gap):
+            out["mixed" if mode else "fp64"]["learned_minus_sad_d1_energy_min_mean_max"] = \
+                [float(gap.min()), float(gap.mean()), float(gap.max())]
+    return out
+
+
 def _generate_shard(a) -> int:
     """Worker: converge clusters [i0, i1) until the deadline, save (X, T, M) to --shard-out."""
     from . import learned as lg
@@ -139,12 +165,22 @@ def main(argv=None):
     ap.add_argument("--mixed-only", action="store_true", help="evaluate the mixed-precision SCF only")
     ap.add_argument("--guard", default="error", choices=["error", "energy"], help="learned-guess acceptance rule")
     ap.add_argument("--report", type=Path, default=None)
+    ap.add_argument("--merge", type=Path, nargs="+", default=None,
+                    help="join the rows of these evaluation reports into --report, with a summary")
     ap.add_argument("--shard", type=int, nargs=2, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--shard-out", type=Path, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--deadline", type=float, default=None, help=argparse.SUPPRESS)
     a = ap.parse_args(argv)
     if a.shard is not None:
         return _generate_shard(a)
+
+    if a.merge:
+        parts = [json.loads(p.read_text()) for p in a.merge]
+        rows = [r for p in parts for r in p["test"]]
+        a.report.write_text(json.dumps({"model": parts[0].get("model"), "meta": parts[0].get("meta", {}),
+                                        "guard": parts[0].get("guard"), "summary": summarize(rows), "test": rows},
+                                       indent=1))
+        return 0
 
     if a.eval_only:
         from . import learned as lg
@@ -154,7 +190,7 @@ def main(argv=None):
             rows += evaluate(net, a.test_n, a.test_seeds, prec_mixed=True, guard=a.guard)
         if a.report:
             a.report.write_text(json.dumps({"model": str(a.model), "meta": getattr(net, "meta", {}), "guard": a.guard,
-                                            "test": rows}, indent=1))
+                                            "summary": summarize(rows), "test": rows}, indent=1))
         return 0
 
     if a.out is None:
