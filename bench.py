@@ -192,13 +192,13 @@ def main_scf(args):
         out = rs.rhf(r, S, H, enuc, mol.nelectron, None, rs.default_ctrl(jk_prec=jk.PREC_FP64), c_guess=c1)
         return out, {"guess": "sad", "e_guess": e0}
 
-    def mixed():
+    def mixed(guard=args.guard, increment_prec=args.increment_prec):
         ctrl = rs.mixed_ctrl(jk_prec=mixed_prec, pseudo_diag_err=args.pseudo_diag_err,
-                             jk_prec_increment=-1 if args.increment_prec == "none" else jk.PREC_NAMES[args.increment_prec])
+                             jk_prec_increment=-1 if increment_prec == "none" else jk.PREC_NAMES[increment_prec])
         if not args.mixed_prec.startswith("bf16"):
             ctrl.jk_prec_start = -1                    # FP32 throughout the low-precision phase
-        g = lg.learned_scf(net, mol, r, S, H, ctrl, guard=args.guard)
-        return g.outcome, {"guess": g.kind, "guard": args.guard, "e_learned": g.e_learned, "e_sad": g.e_sad,
+        g = lg.learned_scf(net, mol, r, S, H, ctrl, guard=guard)
+        return g.outcome, {"guess": g.kind, "guard": guard, "e_learned": g.e_learned, "e_sad": g.e_sad,
                            "err_learned": g.err_learned, "seconds": g.seconds,
                            "incremental_iterations": g.outcome.incremental_iterations,
                            "precisions": [int(x["prec"]) for x in g.outcome.log],
@@ -241,6 +241,21 @@ def main_scf(args):
     torch.cuda.synchronize()
     mixed_sad = {"scf_s": time.perf_counter() - a, "iterations": oms.iterations, "energy": oms.energy,
                  "converged": oms.converged, "precisions": [int(x["prec"]) for x in oms.log]}
+    # outside the timed region: the other acceptance rule and the tail without increments, each
+    # timed on its second run, as ratios to the timed plain median
+    variants = {}
+    other = "energy" if args.guard == "error" else "error"
+    for name, kw in ((f"guard_{other}", {"guard": other}),
+                     ("fp64_tail_without_increments", {"increment_prec": "none"})):
+        for _ in range(2):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            ov, iv = mixed(**kw)
+            torch.cuda.synchronize()
+            tv = time.perf_counter() - a
+        variants[name] = {"scf_s": tv, "ratio": t_plain / tv, "iterations": ov.iterations,
+                          "fp64_iterations": ov.fp64_iterations, "energy": ov.energy, "dE_vs_plain": ov.energy - op.energy,
+                          "guard": iv["guard"], "guess": iv["guess"], "sad_energy_s": iv["seconds"]["sad_energy"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -289,6 +304,7 @@ def main_scf(args):
             "dE_mixed_minus_plain": om.energy - op.energy,
             "all_steps_plain_s": tp, "all_steps_mixed_s": tm,
             "mixed_from_sad_untimed": mixed_sad,
+            "variants_untimed": variants,
             "setup_s": {"int1e": t1 - t0, "b_build": t2 - t1, "b_build_parts": binfo, "sad_atoms_and_model": t3 - t2},
             "kernels": {f"{p}:{ph}": {"ms": ms, "launches": k, "work": w} for (p, ph), (ms, k, w) in
                         detail_prof.items() if k},

@@ -245,5 +245,11 @@ def test_scf_error_paths():
         with pytest.raises(jk.RJKError) as e:
             rs.rhf(r, S, T + V, 0.0, 10, ctrl=rs.default_ctrl(switch_mode=5))
         assert e.value.code == 1
+        with pytest.raises(jk.RJKError) as e:
+            rs.rhf(r, S, T + V, 0.0, 10, ctrl=rs.default_ctrl(jk_prec_increment=jk.PREC_FP64))
+        assert e.value.code == 1          # increments must be at a low precision
+        with pytest.raises(jk.RJKError) as e:
+            rs.rhf(r, S, T + V, 0.0, 10, ctrl=rs.default_ctrl(jk_prec_increment=jk.PREC_BF16X2))
+        assert e.value.code == 4          # no BF16 planes for the increments
         out = rs.rhf(r, S, T + V, m.nuclear_repulsion(), 10)
-        assert out.converged
+        assert out.converged and out.incremental_iterations == 0 and out.guess_error > 0
