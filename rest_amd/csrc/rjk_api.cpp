@@ -73,6 +73,18 @@ PhaseScope::~PhaseScope() {
     ctx->ev_pending.push_back(ev);
 }
 
+int solver_handle(rjk_ctx* ctx, rocblas_handle* h) {
+    if (!ctx->solver) {
+        rocblas_handle hh;
+        if (rocblas_create_handle(&hh) != rocblas_status_success)
+            return fail(ctx, RJK_ERR_NOSOLVER, "rocblas_create_handle failed");
+        ctx->solver = hh;
+    }
+    rocblas_set_stream((rocblas_handle)ctx->solver, ctx->stream);
+    *h = (rocblas_handle)ctx->solver;
+    return RJK_OK;
+}
+
 }  // namespace rjk
 
 using namespace rjk;
@@ -114,6 +126,7 @@ extern "C" int rjk_destroy(rjk_ctx* ctx) {
     if (ctx->b32) (void)hipFree(ctx->b32);
     if (ctx->bpl) (void)hipFree(ctx->bpl);
     if (ctx->scf_x) (void)hipFree(ctx->scf_x);
+    if (ctx->scf_ws) (void)hipFree(ctx->scf_ws);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->solver) rocblas_destroy_handle((rocblas_handle)ctx->solver);
     for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -563,17 +576,6 @@ extern "C" int rjk_vk_upper_with_ri_v_sync(rjk_ctx* ctx, const double* eigv, siz
 
 namespace {
 
-int solver_handle(rjk_ctx* ctx, rocblas_handle* h) {
-    if (!ctx->solver) {
-        rocblas_handle hh;
-        if (rocblas_create_handle(&hh) != rocblas_status_success)
-            return fail(ctx, RJK_ERR_NOSOLVER, "rocblas_create_handle failed");
-        ctx->solver = hh;
-    }
-    rocblas_set_stream((rocblas_handle)ctx->solver, ctx->stream);
-    *h = (rocblas_handle)ctx->solver;
-    return RJK_OK;
-}
 
 // Density-only K on device memory: each spin's D (dm_dev, column-major) is symmetrised,
 // eigen-decomposed (rocSOLVER dsyevd), and the eigenpairs with |λ| > kEigRelCut·max|λ| go

@@ -11,6 +11,8 @@
 
 #include "../../include/rest_jk.h"
 
+struct _rocblas_handle;   // rocblas_handle is _rocblas_handle*
+
 namespace rjk {
 
 constexpr size_t kFp32Chain = 4096;   // products per FP32 accumulator in the K SYRK before FP64
@@ -47,7 +49,9 @@ struct rjk_ctx {
     size_t ws_limit = size_t(4) << 30;
     void* ws = nullptr;       // grow-only workspace
     size_t ws_size = 0;
-    void* solver = nullptr;   // rocblas_handle, created on first density-only K
+    void* solver = nullptr;   // rocblas_handle, created on first use (density-only K, SCF driver)
+    void* scf_ws = nullptr;   // SCF driver buffers, grow-only, reused by every driver call
+    size_t scf_ws_bytes = 0;
     double* scf_x = nullptr;          // SCF driver: orthogonaliser S^-1/2 of the last S seen
     std::vector<double> scf_s;        // that S (host copy, compared before reuse)
     std::string err;
@@ -65,6 +69,8 @@ namespace rjk {
 
 // ---- error helpers ------------------------------------------------------
 int fail(rjk_ctx* ctx, int code, const std::string& msg);
+// the context's rocBLAS/rocSOLVER handle (rocblas_handle), bound to ctx->stream
+int solver_handle(rjk_ctx* ctx, _rocblas_handle** h);
 #define RJK_HIP(ctx, call)                                                        \
     do {                                                                          \
         hipError_t e_ = (call);                                                   \

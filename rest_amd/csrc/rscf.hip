@@ -262,27 +262,38 @@ bool small_inverse(std::vector<double> a, int n, std::vector<double>* inv) {
     return true;
 }
 
+// SCF buffers carved from the context's grow-only arena (ctx->scf_ws), 256-byte aligned.
+// A call whose buffers do not fit allocates the rest one by one and, at its end, grows the
+// arena to its whole demand, so later calls of that size allocate nothing.  Calls on one
+// context are serialised (one host thread, one stream), so the arena is never shared.
 struct DevMem {
-    std::vector<void*> ptrs;
+    rjk_ctx* ctx;
+    size_t used = 0;                   // bytes requested so far
+    std::vector<void*> extra;          // buffers beyond the arena
+    explicit DevMem(rjk_ctx* c) : ctx(c) {}
     ~DevMem() {
-        for (void* p : ptrs) (void)hipFree(p);
+        for (void* p : extra) (void)hipFree(p);
+        if (used > ctx->scf_ws_bytes) {   // grow for the next call (hipFree waits for the device)
+            if (ctx->scf_ws) (void)hipFree(ctx->scf_ws);
+            ctx->scf_ws = nullptr;
+            ctx->scf_ws_bytes = 0;
+            if (hipMalloc(&ctx->scf_ws, used) == hipSuccess) ctx->scf_ws_bytes = used;
+            else (void)hipGetLastError();
+        }
     }
-    double* get(rjk_ctx* ctx, size_t n, int* rc) {
+    double* get(rjk_ctx*, size_t n, int* rc) {
+        const size_t bytes = (n * sizeof(double) + 255) / 256 * 256;
+        const size_t off = used;
+        used += bytes;
+        if (used <= ctx->scf_ws_bytes) return (double*)((char*)ctx->scf_ws + off);
         void* p = nullptr;
-        if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) {
+        if (hipMalloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             *rc = rjk::fail(ctx, RJK_ERR_NOMEM, "SCF device allocation failed");
             return nullptr;
         }
-        ptrs.push_back(p);
+        extra.push_back(p);
         return (double*)p;
-    }
-};
-
-struct BlasHandle {
-    rocblas_handle h = nullptr;
-    ~BlasHandle() {
-        if (h) rocblas_destroy_handle(h);
     }
 };
 
@@ -379,13 +390,11 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     const double t_start = now();
     SCF_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;
-    BlasHandle bh;
-    SCF_BLAS(rocblas_create_handle(&bh.h));
-    SCF_BLAS(rocblas_set_stream(bh.h, st));
-    rocblas_handle h = bh.h;
+    rocblas_handle h = nullptr;
+    SCF_RC(rjk::solver_handle(ctx, &h));
 
     int rc = RJK_OK;
-    DevMem mem;
+    DevMem mem(ctx);
     double* S = mem.get(ctx, n2, &rc);
     double* H = mem.get(ctx, n2, &rc);
     double* Bw = mem.get(ctx, n2, &rc);     // eigensolver operand
@@ -1079,11 +1088,10 @@ extern "C" int rscf_mcweeny(rjk_ctx* ctx, const double* S_h, const double* D_h, 
     const size_t n = ctx->nbf, n2 = n * n;
     SCF_HIP(hipSetDevice(ctx->device));
     const hipStream_t st = ctx->stream;
-    BlasHandle bh;
-    SCF_BLAS(rocblas_create_handle(&bh.h));
-    SCF_BLAS(rocblas_set_stream(bh.h, st));
+    rocblas_handle h = nullptr;
+    SCF_RC(rjk::solver_handle(ctx, &h));
     int rc = RJK_OK;
-    DevMem mem;
+    DevMem mem(ctx);
     double* S = mem.get(ctx, n2, &rc);
     double* P = mem.get(ctx, n2, &rc);
     double* T1 = mem.get(ctx, n2, &rc);
@@ -1099,7 +1107,7 @@ extern "C" int rscf_mcweeny(rjk_ctx* ctx, const double* S_h, const double* D_h, 
     const double one = 1.0, zero = 0.0;
     const rocblas_int ni = (rocblas_int)n;
     auto gemm = [&](const double* A, const double* B, double* C) -> int {
-        SCF_BLAS(rocblas_dgemm(bh.h, rocblas_operation_none, rocblas_operation_none, ni, ni, ni, &one, A, ni, B, ni,
+        SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, ni, ni, ni, &one, A, ni, B, ni,
                                &zero, C, ni));
         return RJK_OK;
     };
