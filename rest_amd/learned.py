@@ -135,7 +135,7 @@ def sph_rotations(l: int, Rs: np.ndarray) -> np.ndarray:
     P = _sample_points()
     if l not in _PINV:
         _PINV[l] = np.linalg.pinv(_eval_rows(l, P))
-    PR = np.einsum("pi,kij->kpj", P, Rs)              # rows (Rᵀ p)ᵀ = pᵀ R
+    PR = np.matmul(P[None], Rs)                        # rows (Rᵀ p)ᵀ = pᵀ R
     ex = _cart_exps(l)
     mono = np.stack([PR[..., 0] ** a * PR[..., 1] ** b * PR[..., 2] ** c for a, b, c in ex], axis=-1)
     return _PINV[l] @ (mono @ harmonic_rows(l).T)
