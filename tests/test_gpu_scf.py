@@ -171,13 +171,14 @@ def test_pseudo_diagonalisation_in_low_precision_iterations(nspin):
     assert om.iterations <= ref.iterations + 2
 
 
-@pytest.mark.parametrize("nspin,pinc", [(1, jk.PREC_BF16X2), (1, jk.PREC_BF16X3), (2, jk.PREC_BF16X2),
-                                        (1, jk.PREC_FP32)])
-def test_incremental_fp64_builds_after_the_switch(nspin, pinc):
+@pytest.mark.parametrize("nspin,pinc,mode", [(1, jk.PREC_BF16X2, rs.SWITCH_RESTART), (1, jk.PREC_BF16X3, rs.SWITCH_RESTART),
+                                             (2, jk.PREC_BF16X2, rs.SWITCH_RESTART), (1, jk.PREC_FP32, rs.SWITCH_RESTART),
+                                             (1, jk.PREC_BF16X2, rs.SWITCH_SHIFT)])
+def test_incremental_fp64_builds_after_the_switch(nspin, pinc, mode):
     """After the switch to FP64 the first FP64 build is the reference and later builds add J/K of
     D − D_ref at a low precision; the converged energy stays the FP64 one and the returned orbitals
     are canonical (an exact diagonalisation of the last Fock matrix)."""
-    say(f"(H2O)4 ladder with incremental FP64 builds at prec {pinc}, nspin {nspin}")
+    say(f"(H2O)4 ladder with incremental FP64 builds at prec {pinc}, nspin {nspin}, switch mode {mode}")
     m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
     S, T, V = m.int1e()
     H = T + V
@@ -191,6 +192,7 @@ def test_incremental_fp64_builds_after_the_switch(nspin, pinc):
         ctrl = tight()
         ctrl.jk_prec, ctrl.jk_prec_start = jk.PREC_BF16X3, jk.PREC_BF16X2
         ctrl.jk_prec_increment = pinc
+        ctrl.switch_mode = mode
         om = run(ctrl)
     precs = [int(x["prec"]) for x in om.log]
     say(f"  E {om.energy:.12f} vs {o64.energy:.12f} (dE {om.energy - o64.energy:.2e}); iterations {om.iterations}, "
