@@ -553,8 +553,9 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     // (1982)): the current orbitals A[s] turn F into Fmo = Aᵀ F A; the occupied orbitals take the
     // first-order rotation U = I + X toward the virtual space; A U is made S-orthonormal again by
     // the Cholesky factor of (A U)ᵀ S (A U) = Uᵀ U; ε ≈ diag(Fmo).  Four GEMMs, one Cholesky and one
-    // triangular solve instead of a full eigensolver; used where the Fock matrix carries
-    // low-precision J/K (the FP64 iterations diagonalise exactly).
+    // triangular solve instead of a full eigensolver; used in low-precision iterations below
+    // pseudo_diag_err and in every iteration from an incremental reference build on (other
+    // FP64 iterations diagonalise exactly).
     auto pseudo_diagonalise = [&](const double* Fm, int s) -> int {
         const double t0 = now();
         SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
