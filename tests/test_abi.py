@@ -67,3 +67,19 @@ def test_reference_shaped_builders_return_dummy_when_b_is_absent():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="not built"):
         jk.load_library(tmp_path / "nope.so")
+
+
+def test_scf_control_defaults_and_the_mixed_ladder():
+    """rscf_default_ctrl leaves every acceleration off (plain REST behaviour); scf.mixed_ctrl turns on
+    the ladder, pseudo-diagonalisation and the incremental FP64 tail.  The ctypes mirror of
+    rscf_ctrl must match the C struct field by field for these to round-trip."""
+    import math
+    from rest_amd import scf as rs
+    c = rs.default_ctrl()
+    assert c.jk_prec == jk.PREC_FP64 and c.jk_prec_start == -1 and c.jk_prec_increment == -1
+    assert c.pseudo_diag_err == 0.0 and math.isinf(c.guess_energy_max) and math.isinf(c.guess_err_max)
+    assert c.switch_diis_err == 1e-5 and c.start_switch_err == 1e-3 and c.switch_mode == rs.SWITCH_RESTART
+    m = rs.mixed_ctrl(max_scf_cycle=77)
+    assert (m.jk_prec, m.jk_prec_start, m.jk_prec_increment) == (jk.PREC_BF16X3, jk.PREC_BF16X2, jk.PREC_BF16X2)
+    assert m.pseudo_diag_err == 1e-2 and m.max_scf_cycle == 77 and m.switch_diis_err == 1e-5
+    assert C.sizeof(rs.Ctrl) % 8 == 0 and C.sizeof(rs.Result) % 8 == 0
