@@ -5,12 +5,17 @@
 --mode scf (default): the headline metric, SCF time to 1e-8 Eh for
   (H2O)64 def2-SVP / def2-SVP-JKFIT (water cluster seed 2): plain FP64 from
   REST's SAD guess vs mixed precision from the learned guess, on the same GPU
-  and the same resident B.  Mixed: K on two BF16 planes of B (J on its FP32
-  copy) until the DIIS error max|FDS-SDF| is below 1e-3, then three planes
-  until 1e-5, then FP64; the learned density is kept only if its energy is
-  below that of SAD's first orbital density (both at the start precision).
-  One step = one plain-FP64 SCF plus one mixed+learned SCF, each including
-  its guess; `value` is their time ratio (plain / mixed+learned).
+  and the same resident B.  Mixed (scf.mixed_ctrl): K on two BF16 planes of
+  B (J on its FP32 copy) until the DIIS error max|FDS-SDF| is below 1e-4,
+  then three planes until 1e-5, then one FP64 reference build and
+  FP64-referenced BF16X2 increments, pseudo-diagonalising in between; the
+  learned density is kept when its guess build's max|FDS-SDF| is below
+  learned.GUESS_ERR_MAX (--guard error, default) or when its energy is below
+  that of SAD's first orbital density (--guard energy).  One step = one
+  plain-FP64 SCF plus one mixed+learned SCF, each including its guess;
+  `value` is their time ratio (plain / mixed+learned).  The other guard, the
+  FP64 tail without increments and the mixed ladder from SAD are timed once
+  each outside the timed region and reported under detail.
   Integrals, the B build and the atomic SAD densities are outside the timed
   region and reported separately.  --mixed-prec fp32 runs the mixed SCF on
   the FP32 copy of B instead.

@@ -82,4 +82,5 @@ def test_scf_control_defaults_and_the_mixed_ladder():
     m = rs.mixed_ctrl(max_scf_cycle=77)
     assert (m.jk_prec, m.jk_prec_start, m.jk_prec_increment) == (jk.PREC_BF16X3, jk.PREC_BF16X2, jk.PREC_BF16X2)
     assert m.pseudo_diag_err == 1e-2 and m.max_scf_cycle == 77 and m.switch_diis_err == 1e-5
+    assert m.start_switch_err == 1e-4
     assert C.sizeof(rs.Ctrl) % 8 == 0 and C.sizeof(rs.Result) % 8 == 0
