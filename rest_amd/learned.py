@@ -679,9 +679,9 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
     d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
     t3 = time.perf_counter()
     enuc = mol.nuclear_repulsion()
-    e_s, c1_s = None, None
+    e_s = None
     if guard == "energy":
-        _, _, c1_s, _, e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True)
+        e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True, want_outputs=False)[4]
     t4 = time.perf_counter()
     ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
     out, e_l, err_l = None, float("inf"), None
@@ -696,11 +696,8 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
         if out.guess_rejected or not np.isfinite(e_l):
             out = None
     kind = "learned" if out is not None else "sad"
-    if out is None:
-        if c1_s is not None:
-            out = rs.rhf(r, S, H, enuc, mol.nelectron, None, ctrl, c_guess=c1_s)
-        else:
-            out = rs.rhf(r, S, H, enuc, mol.nelectron, d_sad, ctrl)
+    if out is None:   # SAD, with REST's guess step inside the driver
+        out = rs.rhf(r, S, H, enuc, mol.nelectron, d_sad, ctrl)
     t5 = time.perf_counter()
     secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "scf": t5 - t4}
     return LearnedScf(out, kind, e_l, e_s, ne, secs, err_l)

@@ -172,26 +172,31 @@ def uhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nalpha: in
     return _run(ri, S, H, enuc, (nalpha, nbeta), d_guess, ctrl, want_orbitals)
 
 
-def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PREC_FP64, want_e1: bool = False):
+def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PREC_FP64, want_e1: bool = False,
+               want_outputs: bool = True):
     """REST's density-guess step (rscf_guess_step): returns (E(D0), D1, C, ε), plus E(D1) when
-    want_e1; nel = (nelec,) or (nα, nβ)."""
+    want_e1; nel = (nelec,) or (nα, nβ).  With want_outputs=False D1, C and ε stay on the device
+    and come back as None (the energies only)."""
     lib = _bind(jk.load_library())
     n = ri.nbf
     ns = len(nel)
     d0s = [d0] if ns == 1 else list(d0)
     d0f = np.ascontiguousarray(np.stack([_sym(d) for d in d0s]))
-    d1 = np.empty((ns, n, n))
-    C_ = np.empty((ns, n, n))
-    eps = np.empty((ns, n))
+    d1 = np.empty((ns, n, n)) if want_outputs else None
+    C_ = np.empty((ns, n, n)) if want_outputs else None
+    eps = np.empty((ns, n)) if want_outputs else None
+    ptr = lambda a: None if a is None else a.ctypes.data
     nl = (C.c_size_t * 2)(*(list(nel) + [0] * (2 - ns)))
     e = C.c_double(0.0)
     e1 = C.c_double(0.0)
     Sf, Hf = _sym(S), _sym(H)      # kept alive across the call
     rc = lib.rscf_guess_step(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, d0f.ctypes.data, int(prec),
-                             C.byref(e), d1.ctypes.data, C_.ctypes.data, eps.ctypes.data,
-                             C.byref(e1) if want_e1 else None)
+                             C.byref(e), ptr(d1), ptr(C_), ptr(eps), C.byref(e1) if want_e1 else None)
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
+    if not want_outputs:
+        out = (e.value, None, None, None)
+        return out + (e1.value,) if want_e1 else out
     C_ = C_.transpose(0, 2, 1)     # column-major outputs; d1 is symmetric
     out = (e.value, (d1[0] if ns == 1 else d1), (C_[0] if ns == 1 else C_), (eps[0] if ns == 1 else eps))
     return out + (e1.value,) if want_e1 else out
