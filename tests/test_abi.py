@@ -54,6 +54,7 @@ def test_null_context_calls_are_rejected():
     assert lib.rjk_num_basis(None) == 0
     assert lib.rjk_vj_upper_with_rimatr_v_sync(None, None, 1, 1.0, 0, None) == 1
     assert lib.rjk_vk_upper_with_ri_v_sync(None, None, 1, None, None, 1, 1.0, 0, None) == 1
+    assert lib.rjk_vk_upper_dm_dev(None, None, 1, 1.0, 0, None) == 1
 
 
 def test_reference_shaped_builders_return_dummy_when_b_is_absent():
