@@ -14,8 +14,8 @@
   that of SAD's first orbital density (--guard energy).  One step = one
   plain-FP64 SCF plus one mixed+learned SCF, each including its guess;
   `value` is their time ratio (plain / mixed+learned).  The other guard, the
-  FP64 tail without increments and the mixed ladder from SAD are timed once
-  each outside the timed region and reported under detail.
+  FP64 tail without increments, both together and the mixed ladder from SAD
+  are timed once each outside the timed region and reported under detail.
   Integrals, the B build and the atomic SAD densities are outside the timed
   region and reported separately.  --mixed-prec fp32 runs the mixed SCF on
   the FP32 copy of B instead.
@@ -251,7 +251,8 @@ def main_scf(args):
     variants = {}
     other = "energy" if args.guard == "error" else "error"
     for name, kw in ((f"guard_{other}", {"guard": other}),
-                     ("fp64_tail_without_increments", {"increment_prec": "none"})):
+                     ("fp64_tail_without_increments", {"increment_prec": "none"}),
+                     ("guard_energy_and_fp64_tail_without_increments", {"guard": "energy", "increment_prec": "none"})):
         for _ in range(2):
             torch.cuda.synchronize()
             a = time.perf_counter()
