@@ -30,9 +30,11 @@ Representation
 
 Guess: D = D_sph + predicted blocks (pairs beyond R_CUT keep zero),
 symmetrised, scaled to the molecule's electron count (tr D S = N),
-McWeeny-purified (rscf_mcweeny), and accepted only if its energy E(D) is
-below the energy of SAD's first orbital density (REST's guess step on the SAD
-density, then rscf_energy); otherwise the SAD guess is used.
+McWeeny-purified (rscf_mcweeny), and accepted by one of two guards, else the
+SAD guess is used: the guess build's own self-consistency error
+max|F D S - S D F| below GUESS_ERR_MAX (learned_scf's default), or its energy
+E(D) below the energy of SAD's first orbital density (REST's guess step on
+the SAD density; learned_guess, and learned_scf with guard="energy").
 
 The network runs on the device through include/rest_lg.h (BF16 matrix-core
 GEMMs, FP32 master weights, Adam); torch only provides device buffers.
