@@ -235,7 +235,7 @@ def main_scf(args):
     op, ip, om, im = runs[-1]
     t_plain, t_mixed = float(np.median(tp)), float(np.median(tm))
     roofline = roofline_from_detail(detail_prof)
-    # outside the timed region: the same mixed ladder from SAD (REST's guess step inside the driver),
+    # outside the timed region: the same mixed ladder from SAD (REST's guess step inside rscf_rhf),
     # to separate what the learned guess and what the ladder contribute
     torch.cuda.synchronize()
     a = time.perf_counter()

@@ -698,7 +698,7 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
         if out.guess_rejected or not np.isfinite(e_l):
             out = None
     kind = "learned" if out is not None else "sad"
-    if out is None:   # SAD, with REST's guess step inside the driver
+    if out is None:   # SAD, with REST's guess step inside rscf_rhf
         out = rs.rhf(r, S, H, enuc, mol.nelectron, d_sad, ctrl)
     t5 = time.perf_counter()
     secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "scf": t5 - t4}
