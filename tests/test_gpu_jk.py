@@ -202,6 +202,22 @@ def test_density_only_device_route():
         assert maxabs(host, orc.vk_upper_dm(b, True, [d], 1, 0.5)) <= TOL
         with pytest.raises(jk.RJKError):
             ri.vk_dm([d + skew], 1, 0.5)
+        # two spins in one call (spin-major device buffers), FP64 and three BF16 planes
+        d2 = rng.standard_normal((nbf, nbf))
+        d2 = d2 @ d2.T / nbf
+        ref2 = orc.vk_upper_dm(b, True, [d, d2], 2, -1.0)
+        both = torch.from_numpy(np.ascontiguousarray(np.stack([d.T, d2.T]))).to(dev)
+        out2 = torch.empty((2, npair), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        ri.vk_dm_dev(both.data_ptr(), 2, -1.0, out2.data_ptr())
+        ri.synchronize()
+        assert maxabs(out2.cpu().numpy(), ref2) <= TOL
+        ri.set_rimatr(b, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        ri.vk_dm_dev(both.data_ptr(), 2, -1.0, out2.data_ptr(), prec=jk.PREC_BF16X3)
+        ri.synchronize()
+        k3 = out2.cpu().numpy()
+        for s_ in range(2):
+            assert np.max(np.abs(k3[s_] - ref2[s_])) / np.abs(ref2[s_]).max() < 2e-6
 
 
 def test_workspace_chunking_matches_single_pass():
