@@ -53,6 +53,18 @@ def test_pentacene_device_scf_matches_rest_and_oracle():
     assert out2.converged and abs(out2.energy - ref) <= 5e-9
 
 
+def test_pentacene_mixed_ladder_matches_rest():
+    """The mixed ladder (scf.mixed_ctrl: BF16X2 -> BF16X3 -> FP64 reference + BF16X2 increments,
+    pseudo-diagonalisation) from REST's SAD guess on REST's own pentacene example: the converged
+    energy is REST's printed one (examples/PEN/RIHF/rest.out:85) to its 1e-8 Eh convergence."""
+    say("pentacene STO-3G/def2-SV(P)-JKFIT, mixed ladder from SAD (REST defaults)")
+    out, _ = rs.run_molecule(pentacene(), prec=jk.PREC_BF16X3, ctrl=rs.mixed_ctrl())
+    precs = [int(x["prec"]) for x in out.log]
+    say(f"  E {out.energy:.10f} iters {out.iterations} (incremental {out.incremental_iterations}), precisions {precs}")
+    assert out.converged and abs(out.energy - (-831.0241601906)) < 1e-8
+    assert precs[0] == jk.PREC_BF16X2 and precs[-1] == jk.PREC_FP64 and out.incremental_iterations >= 1
+
+
 def test_identical_b_device_scf_vs_oracle_scf():
     """Same B (read back from HBM), same S, H: device SCF vs the oracle SCF loop, tight thresholds."""
     from oracle import scf_oracle as so
