@@ -152,9 +152,12 @@ int rjk_vk_upper_occ_dev(rjk_ctx* ctx, const double* eigv_dev, size_t num_state,
 /* vk_upper_with_rimatr_use_dm_only_sync (src/scf_io/mod.rs:2703-2756) and
  * vk_upper_with_ri_v_use_dm_only_sync (:2640-2701):
  *   K_s = scaling_factor · upper(Σ_P B_P D_s B_P)
- * D must be symmetric (max|D−Dᵀ| <= 1e-10·max|D|, else RJK_ERR_ARG).  It is
+ * for any D, read in full (REST's _dsymm(B_P, D) then _dsymm(·, B_P), :2730-2732,
+ * :2675-2677).  A symmetric D (max|D−Dᵀ| <= 1e-10·max|D| for every spin) is
  * factorised on the device as D = U Λ Uᵀ and the occupied route runs with
- * signed weights Λ. */
+ * signed weights Λ at `prec`; any other D takes the general route at FP64,
+ * K = upper(Σ_P (B_P D)(B_P)ᵀ) through the FP64 half-transform and one DGEMM
+ * per block of auxiliary columns (about 6·nbf³ FLOP per column). */
 int rjk_vk_upper_with_rimatr_use_dm_only_sync(rjk_ctx* ctx, const double* dm,
                                               size_t spin_channel, double scaling_factor,
                                               int prec, double* vk_upper);
@@ -162,8 +165,8 @@ int rjk_vk_upper_with_ri_v_use_dm_only_sync(rjk_ctx* ctx, const double* dm,
                                             size_t spin_channel, double scaling_factor,
                                             int prec, double* vk_upper);
 /* Device form of the density-only route: dm_dev (spin_channel × nbf × nbf, column-major) and
- * vk_dev (spin_channel × npair) in HBM.  Each D is symmetrised, (D + Dᵀ)/2, instead of being
- * checked; non-finite eigenvalues give RJK_ERR_ARG. */
+ * vk_dev (spin_channel × npair) in HBM, the same semantics and routes as above; a D holding
+ * NaN/Inf gives RJK_ERR_NONFINITE.  Synchronises the stream (the route is chosen on the host). */
 int rjk_vk_upper_dm_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel,
                         double scaling_factor, int prec, double* vk_dev);
 

@@ -35,25 +35,27 @@ int fail(rjk_ctx* ctx, int code, const std::string& msg) {
     return code;
 }
 
-int ws_get(rjk_ctx* ctx, size_t bytes, void** out) {
-    if (bytes > ctx->ws_size) {
+int buf_get(rjk_ctx* ctx, void** buf, size_t* size, size_t bytes, void** out) {
+    if (bytes > *size) {
         RJK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->ws) (void)hipFree(ctx->ws);
-        ctx->ws = nullptr;
-        ctx->ws_size = 0;
+        if (*buf) (void)hipFree(*buf);
+        *buf = nullptr;
+        *size = 0;
         size_t sz = round_up(bytes, size_t(1) << 20);
-        if (hipMalloc(&ctx->ws, sz) != hipSuccess) {
+        if (hipMalloc(buf, sz) != hipSuccess) {
             (void)hipGetLastError();
-            ctx->ws = nullptr;
-            return fail(ctx, RJK_ERR_NOMEM, "workspace allocation of " + std::to_string(sz) + " bytes failed");
+            *buf = nullptr;
+            return fail(ctx, RJK_ERR_NOMEM, "device allocation of " + std::to_string(sz) + " bytes failed");
         }
-        ctx->ws_size = sz;
+        *size = sz;
     }
-    *out = ctx->ws;
+    *out = *buf;
     return RJK_OK;
 }
 
-PhaseScope::PhaseScope(rjk_ctx* c, int phase, int prec, double work) : ctx(c), on(c->prof) {
+int ws_get(rjk_ctx* ctx, size_t bytes, void** out) { return buf_get(ctx, &ctx->ws, &ctx->ws_size, bytes, out); }
+
+PhaseScope::PhaseScope(rjk_ctx* c, int phase, int prec, double work, bool allow) : ctx(c), on(allow && c->prof) {
     if (!on) return;
     if (!ctx->ev_free.empty()) {
         ev = ctx->ev_free.back();
@@ -128,6 +130,8 @@ extern "C" int rjk_destroy(rjk_ctx* ctx) {
     if (ctx->scf_x) (void)hipFree(ctx->scf_x);
     if (ctx->scf_ws) (void)hipFree(ctx->scf_ws);
     if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->io) (void)hipFree(ctx->io);
+    if (ctx->aux) (void)hipFree(ctx->aux);
     if (ctx->solver) rocblas_destroy_handle((rocblas_handle)ctx->solver);
     for (auto& e : ctx->ev_pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto& e : ctx->ev_free) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -287,7 +291,7 @@ extern "C" int rjk_get_rimatr(rjk_ctx* ctx, double* rimatr) {
 // ---------------------------------------------------------------------------
 // J
 // ---------------------------------------------------------------------------
-namespace {
+namespace rjk {
 
 // The B copy a call reads.  J: the BF16 precisions run on the FP32 copy when it is
 // stored, else on the FP64 copy (*jprec receives the precision J runs at).  K: the
@@ -316,7 +320,67 @@ int check_common(rjk_ctx* ctx, size_t spin_channel, int prec, bool for_k, const 
     return RJK_OK;
 }
 
+int j_source(rjk_ctx* ctx, size_t spin_channel, int prec, const void** bptr, int* jprec) {
+    return check_common(ctx, spin_channel, prec, false, bptr, jprec);
+}
+
+namespace {
+// [w | γ partials | γ | J partials] of one J call
+struct JLayout {
+    size_t nchunk = 0, nsplit = 0;
+    Carve cv;
+    size_t iw = 0, ig = 0, iG = 0, ij = 0;
+};
+JLayout j_layout(const rjk_ctx* ctx, int ns, int prec) {
+    JLayout L;
+    j_plan(ctx->ldb, ctx->naux, prec, &L.nchunk, &L.nsplit);
+    L.iw = L.cv.add(ns * ctx->ldb * sizeof(double));
+    L.ig = L.cv.add(L.nchunk * ns * ctx->naux * sizeof(double));
+    L.iG = L.cv.add(ns * ctx->naux * sizeof(double));
+    L.ij = L.cv.add(L.nsplit * ns * ctx->ldb * sizeof(double));
+    return L;
+}
 }  // namespace
+
+size_t j_ws_bytes(const rjk_ctx* ctx, int nspin, int prec) { return j_layout(ctx, nspin, prec).cv.total; }
+
+// Every workspace element a J call reads is written earlier in the same call, so the
+// workspace needs no clearing.
+int j_run(rjk_ctx* ctx, const void* b, int prec, const double* dm_dev, int ns, double scale, int full_dm,
+          double* vj_dev, void* ws, bool timed) {
+    const JLayout L = j_layout(ctx, ns, prec);
+    char* base = (char*)ws;
+    double* w = (double*)(base + L.cv.off[L.iw]);
+    double* gpart = (double*)(base + L.cv.off[L.ig]);
+    double* gamma = (double*)(base + L.cv.off[L.iG]);
+    double* jpart = (double*)(base + L.cv.off[L.ij]);
+    hipStream_t st = ctx->stream;
+    {
+        PhaseScope ps(ctx, 4, 0, 0.0, timed);
+        RJK_HIP(ctx, launch_dm_weights(st, dm_dev, ctx->nbf, ctx->npair, ctx->ldb, ns, full_dm, w));
+    }
+    const double sb = prec == RJK_PREC_FP64 ? 8.0 : 4.0;
+    const double jpass = (double)ctx->npair * ctx->naux * sb + 8.0 * ns * ctx->npair;   // one pass over B
+    {
+        PhaseScope ps(ctx, 0, prec, jpass, timed);
+        RJK_HIP(ctx, launch_jgamma(st, prec, b, ctx->ldb, ctx->naux, ns, w, L.nchunk, gpart));
+    }
+    {
+        PhaseScope ps(ctx, 4, 0, 0.0, timed);
+        RJK_HIP(ctx, launch_gamma_reduce(st, gpart, L.nchunk, ns, ctx->naux, gamma));
+    }
+    {
+        PhaseScope ps(ctx, 1, prec, jpass, timed);
+        RJK_HIP(ctx, launch_jacc(st, prec, b, ctx->ldb, ctx->naux, ns, gamma, L.nsplit, jpart));
+    }
+    {
+        PhaseScope ps(ctx, 4, 0, 0.0, timed);
+        RJK_HIP(ctx, launch_jreduce(st, jpart, L.nsplit, ns, ctx->ldb, ctx->npair, scale, vj_dev));
+    }
+    return RJK_OK;
+}
+
+}  // namespace rjk
 
 extern "C" int rjk_vj_upper_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel, double scaling_factor,
                                 int prec, int full_dm, double* vj_dev) {
@@ -327,56 +391,16 @@ extern "C" int rjk_vj_upper_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_
     if (rc) return rc;
     RJK_HIP(ctx, hipSetDevice(ctx->device));
     const int ns = (int)spin_channel;
-    size_t nchunk, nsplit;
-    j_plan(ctx->ldb, ctx->naux, prec, &nchunk, &nsplit);
-    Carve cv;
-    size_t iw = cv.add(ns * ctx->ldb * sizeof(double));
-    size_t ig = cv.add(nchunk * ns * ctx->naux * sizeof(double));
-    size_t iG = cv.add(ns * ctx->naux * sizeof(double));
-    size_t ij = cv.add(nsplit * ns * ctx->ldb * sizeof(double));
     void* ws = nullptr;
-    rc = ws_get(ctx, cv.total, &ws);
+    rc = ws_get(ctx, j_ws_bytes(ctx, ns, prec), &ws);
     if (rc) return rc;
-    char* base = (char*)ws;
-    double* w = (double*)(base + cv.off[iw]);
-    double* gpart = (double*)(base + cv.off[ig]);
-    double* gamma = (double*)(base + cv.off[iG]);
-    double* jpart = (double*)(base + cv.off[ij]);
-    hipStream_t st = ctx->stream;
-    {
-        PhaseScope ps(ctx, 4);
-        RJK_HIP(ctx, launch_dm_weights(st, dm_dev, ctx->nbf, ctx->npair, ctx->ldb, ns, full_dm, w));
-    }
-    const double sb = prec == RJK_PREC_FP64 ? 8.0 : 4.0;
-    const double jpass = (double)ctx->npair * ctx->naux * sb + 8.0 * ns * ctx->npair;   // one pass over B
-    {
-        PhaseScope ps(ctx, 0, prec, jpass);
-        RJK_HIP(ctx, launch_jgamma(st, prec, b, ctx->ldb, ctx->naux, ns, w, nchunk, gpart));
-    }
-    {
-        PhaseScope ps(ctx, 4);
-        RJK_HIP(ctx, launch_gamma_reduce(st, gpart, nchunk, ns, ctx->naux, gamma));
-    }
-    {
-        PhaseScope ps(ctx, 1, prec, jpass);
-        RJK_HIP(ctx, launch_jacc(st, prec, b, ctx->ldb, ctx->naux, ns, gamma, nsplit, jpart));
-    }
-    {
-        PhaseScope ps(ctx, 4);
-        RJK_HIP(ctx, launch_jreduce(st, jpart, nsplit, ns, ctx->ldb, ctx->npair, scaling_factor, vj_dev));
-    }
-    return RJK_OK;
+    return j_run(ctx, b, prec, dm_dev, ns, scaling_factor, full_dm, vj_dev, ws, true);
 }
 
 namespace {
 
-// host staging buffer separate from the internal workspace
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-};
+// staging of the host-pointer entry points: the context's grow-only io buffer
+int io_get(rjk_ctx* ctx, size_t bytes, void** out) { return buf_get(ctx, &ctx->io, &ctx->io_size, bytes, out); }
 
 int check_host_finite(rjk_ctx* ctx, const double* x, size_t n, const char* what) {
     for (size_t i = 0; i < n; ++i)
@@ -394,10 +418,11 @@ int vj_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scaling_
     int rc = check_host_finite(ctx, dm, spin_channel * n2, "dm");
     if (rc) return rc;
     RJK_HIP(ctx, hipSetDevice(ctx->device));
-    DevBuf io;
+    void* io = nullptr;
     const size_t out_n = spin_channel * ctx->npair;
-    RJK_HIP(ctx, hipMalloc(&io.p, (spin_channel * n2 + out_n) * sizeof(double)));
-    double* d_dm = (double*)io.p;
+    rc = io_get(ctx, (spin_channel * n2 + out_n) * sizeof(double), &io);
+    if (rc) return rc;
+    double* d_dm = (double*)io;
     double* d_out = d_dm + spin_channel * n2;
     RJK_HIP(ctx, hipMemcpyAsync(d_dm, dm, spin_channel * n2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     rc = rjk_vj_upper_dev(ctx, d_dm, spin_channel, scaling_factor, prec, full_dm, d_out);
@@ -559,10 +584,11 @@ extern "C" int rjk_vk_upper_with_ri_v_sync(rjk_ctx* ctx, const double* eigv, siz
     rc = check_host_finite(ctx, occupation, no, "occupation");
     if (rc) return rc;
     RJK_HIP(ctx, hipSetDevice(ctx->device));
-    DevBuf io;
+    void* io = nullptr;
     const size_t out_n = spin_channel * ctx->npair;
-    RJK_HIP(ctx, hipMalloc(&io.p, (ne + no + out_n) * sizeof(double)));
-    double* d_e = (double*)io.p;
+    rc = io_get(ctx, (ne + no + out_n) * sizeof(double), &io);
+    if (rc) return rc;
+    double* d_e = (double*)io;
     double* d_o = d_e + ne;
     double* d_out = d_o + no;
     RJK_HIP(ctx, hipMemcpyAsync(d_e, eigv, ne * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -577,10 +603,65 @@ extern "C" int rjk_vk_upper_with_ri_v_sync(rjk_ctx* ctx, const double* eigv, siz
 namespace {
 
 
-// Density-only K on device memory: each spin's D (dm_dev, column-major) is symmetrised,
-// eigen-decomposed (rocSOLVER dsyevd), and the eigenpairs with |λ| > kEigRelCut·max|λ| go
-// through the occupied route with signed weights λ; K_s = scaling_factor · upper(Σ_P B_P D_s B_P)
-// lands in vk_dev (spin_channel × npair).
+// Density-only K of a D that is not symmetric, all spins: K_s = scale · upper(Σ_P B_P D_s B_P)
+// written as Σ_P (B_P D_s)(B_P I)ᵀ.  Both column sets, D's columns and the identity's, go
+// through the FP64 half-transform (M_X = B_P D, M_Y = B_P exactly), and one DGEMM per P-chunk
+// with inner dimension P·nk accumulates G = M_X M_Yᵀ over the chunk.  This is REST's
+// T = B_P D, K_P = T B_P (src/scf_io/mod.rs:2730-2732) with the P-sum moved into the GEMM;
+// it runs at FP64 whatever the requested precision (no SCF density takes it).
+int vk_dm_general(rjk_ctx* ctx, const double* dm_dev, int ns, double scale, double* vk_dev) {
+    const size_t nbf = ctx->nbf, n2 = nbf * nbf;
+    hipStream_t st = ctx->stream;
+    KPlan kp;
+    kp.nbf_pad = round_up(nbf, kTileMN);
+    kp.nk = round_up(nbf, kColPad);
+    kp.col_cnt[0] = kp.nk;
+    const size_t ccol = kp.nbf_pad * kp.nk * sizeof(double);
+    const size_t gbytes = n2 * sizeof(double);
+    const size_t perM = kp.nk * kp.nbf_pad * sizeof(double);   // one M' column block per P
+    const size_t fixed = 2 * round_up(ccol, 256) + round_up(gbytes, 256) + 512;
+    const size_t avail = ctx->ws_limit > fixed ? ctx->ws_limit - fixed : 0;
+    const size_t kmax = (size_t)0x7fffffff / kp.nk;              // rocBLAS inner dimension is an int
+    const size_t chunk = std::min({ctx->naux, std::max<size_t>(1, avail / (2 * perM)), kmax});
+    Carve cv;
+    const size_t ix = cv.add(ccol), iy = cv.add(ccol), ig = cv.add(gbytes), imx = cv.add(chunk * perM),
+                 imy = cv.add(chunk * perM);
+    void* ws = nullptr;
+    int rc = ws_get(ctx, cv.total, &ws);
+    if (rc) return rc;
+    char* base = (char*)ws;
+    double* cx = (double*)(base + cv.off[ix]);
+    double* cy = (double*)(base + cv.off[iy]);
+    double* g = (double*)(base + cv.off[ig]);
+    double* mx = (double*)(base + cv.off[imx]);
+    double* my = (double*)(base + cv.off[imy]);
+    rocblas_handle h;
+    rc = solver_handle(ctx, &h);
+    if (rc) return rc;
+    const double one = 1.0, zero = 0.0;
+    PhaseScope ps(ctx, 4);
+    for (int s = 0; s < ns; ++s) {
+        RJK_HIP(ctx, launch_prep_general(st, dm_dev + s * n2, nbf, kp.nk, kp.nbf_pad, cx, cy));
+        for (size_t P0 = 0; P0 < ctx->naux; P0 += chunk) {
+            const size_t Pc = std::min(chunk, ctx->naux - P0);
+            RJK_HIP(ctx, launch_khalf(st, RJK_PREC_FP64, ctx->b64, ctx->ldb, nbf, P0, Pc, kp, cx, mx));
+            RJK_HIP(ctx, launch_khalf(st, RJK_PREC_FP64, ctx->b64, ctx->ldb, nbf, P0, Pc, kp, cy, my));
+            if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)nbf,
+                              (rocblas_int)nbf, (rocblas_int)(Pc * kp.nk), &one, mx, (rocblas_int)kp.nbf_pad, my,
+                              (rocblas_int)kp.nbf_pad, P0 ? &one : &zero, g, (rocblas_int)nbf) !=
+                rocblas_status_success)
+                return fail(ctx, RJK_ERR_HIP, "vk (density-only, general D): rocblas_dgemm failed");
+        }
+        RJK_HIP(ctx, launch_pack_upper(st, g, nbf, nbf, scale, vk_dev + s * ctx->npair));
+    }
+    return RJK_OK;
+}
+
+// Density-only K on device memory, K_s = scaling_factor · upper(Σ_P B_P D_s B_P) into vk_dev
+// (spin_channel × npair).  A device reduction measures max|D − Dᵀ| against max|D| per spin.
+// When every D is symmetric to kAsymTol, (D + Dᵀ)/2 is eigen-decomposed (rocSOLVER dsyevd) and
+// the eigenpairs with |λ| > kEigRelCut·max|λ| go through the occupied route with signed
+// weights λ; otherwise the general route above reads every D in full.
 int vk_dm_device(rjk_ctx* ctx, const void* b, const double* dm_dev, size_t spin_channel, double scaling_factor,
                  int prec, double* vk_dev) {
     const size_t nbf = ctx->nbf, n2 = nbf * nbf;
@@ -590,15 +671,45 @@ int vk_dm_device(rjk_ctx* ctx, const void* b, const double* dm_dev, size_t spin_
     int rc = solver_handle(ctx, &h);
     if (rc) return rc;
     hipStream_t st = ctx->stream;
-    DevBuf io;
-    size_t io_n = spin_channel * n2 /*vecs*/ + spin_channel * nbf /*lam*/ + nbf /*E*/ + spin_channel * nbf /*sel lam*/;
-    RJK_HIP(ctx, hipMalloc(&io.p, io_n * sizeof(double) + 2 * spin_channel * nbf * sizeof(int) + 256));
-    double* d_vec = (double*)io.p;
-    double* d_lam = d_vec + spin_channel * n2;
-    double* d_E = d_lam + spin_channel * nbf;
-    double* d_sel = d_E + nbf;
-    int* d_idx = (int*)(d_sel + spin_channel * nbf);
-    int* d_info = d_idx + spin_channel * nbf;
+    Carve cv;
+    const size_t i_st = cv.add(2 * kAsymBlocks * spin_channel * sizeof(double));
+    const size_t i_vec = cv.add(spin_channel * n2 * sizeof(double));
+    const size_t i_lam = cv.add(spin_channel * nbf * sizeof(double));
+    const size_t i_E = cv.add(nbf * sizeof(double));
+    const size_t i_sel = cv.add(spin_channel * nbf * sizeof(double));
+    const size_t i_idx = cv.add(spin_channel * nbf * sizeof(int));
+    const size_t i_info = cv.add(sizeof(int));
+    void* aux = nullptr;
+    rc = buf_get(ctx, &ctx->aux, &ctx->aux_size, cv.total, &aux);
+    if (rc) return rc;
+    char* base = (char*)aux;
+    double* d_stats = (double*)(base + cv.off[i_st]);
+    double* d_vec = (double*)(base + cv.off[i_vec]);
+    double* d_lam = (double*)(base + cv.off[i_lam]);
+    double* d_E = (double*)(base + cv.off[i_E]);
+    double* d_sel = (double*)(base + cv.off[i_sel]);
+    int* d_idx = (int*)(base + cv.off[i_idx]);
+    int* d_info = (int*)(base + cv.off[i_info]);
+    // symmetry of every D
+    const unsigned nblk = asym_stats_blocks(nbf);
+    std::vector<double> stats(2 * nblk * spin_channel);
+    for (int s = 0; s < ns; ++s) {
+        RJK_HIP(ctx, launch_asym_stats(st, dm_dev + s * n2, nbf, d_stats + 2 * kAsymBlocks * s));
+        RJK_HIP(ctx, hipMemcpyAsync(stats.data() + 2 * nblk * s, d_stats + 2 * kAsymBlocks * s,
+                                    2 * nblk * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    RJK_HIP(ctx, hipStreamSynchronize(st));
+    bool general = false;
+    for (int s = 0; s < ns; ++s) {
+        double asym = 0, amax = 0;
+        for (unsigned q = 0; q < nblk; ++q) {
+            asym = std::max(asym, stats[2 * nblk * s + 2 * q]);
+            amax = std::max(amax, stats[2 * nblk * s + 2 * q + 1]);
+        }
+        if (!std::isfinite(amax)) return fail(ctx, RJK_ERR_NONFINITE, "dm holds NaN/Inf");
+        if (asym > kAsymTol * std::max(amax, 1e-300)) general = true;
+    }
+    if (general) return vk_dm_general(ctx, dm_dev, ns, scaling_factor, vk_dev);
     std::vector<double> lam(nbf);
     std::vector<int> sel_idx[2];
     std::vector<double> sel_lam[2];
@@ -615,7 +726,7 @@ int vk_dm_device(rjk_ctx* ctx, const void* b, const double* dm_dev, size_t spin_
         if (info != 0) return fail(ctx, RJK_ERR_NOSOLVER, "rocsolver_dsyevd info = " + std::to_string(info));
         double lmax = 0;
         for (double v : lam) {
-            if (!std::isfinite(v)) return fail(ctx, RJK_ERR_ARG, "vk (density-only): dm has non-finite entries");
+            if (!std::isfinite(v)) return fail(ctx, RJK_ERR_NOSOLVER, "vk (density-only): non-finite eigenvalue");
             lmax = std::max(lmax, std::fabs(v));
         }
         for (size_t q = 0; q < nbf; ++q)
@@ -662,27 +773,15 @@ int vk_dm_host(rjk_ctx* ctx, const double* dm, size_t spin_channel, double scali
     int rc = check_common(ctx, spin_channel, prec, true, &b);
     if (rc) return rc;
     const size_t nbf = ctx->nbf, n2 = nbf * nbf;
-    const int ns = (int)spin_channel;
     rc = check_host_finite(ctx, dm, spin_channel * n2, "dm");
     if (rc) return rc;
-    for (int s = 0; s < ns; ++s) {
-        const double* d = dm + s * n2;
-        double amax = 0, asym = 0;
-        for (size_t j = 0; j < nbf; ++j)
-            for (size_t i = 0; i < nbf; ++i) {
-                amax = std::max(amax, std::fabs(d[i + j * nbf]));
-                asym = std::max(asym, std::fabs(d[i + j * nbf] - d[j + i * nbf]));
-            }
-        if (asym > kAsymTol * std::max(amax, 1e-300))
-            return fail(ctx, RJK_ERR_ARG, "vk (density-only): dm is not symmetric (max|D-D^T| = " +
-                                              std::to_string(asym) + ")");
-    }
     RJK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    DevBuf io;
+    void* io = nullptr;
     const size_t out_n = spin_channel * ctx->npair;
-    RJK_HIP(ctx, hipMalloc(&io.p, (spin_channel * n2 + out_n) * sizeof(double)));
-    double* d_dm = (double*)io.p;
+    rc = io_get(ctx, (spin_channel * n2 + out_n) * sizeof(double), &io);
+    if (rc) return rc;
+    double* d_dm = (double*)io;
     double* d_out = d_dm + spin_channel * n2;
     RJK_HIP(ctx, hipMemcpyAsync(d_dm, dm, spin_channel * n2 * sizeof(double), hipMemcpyHostToDevice, st));
     rc = vk_dm_device(ctx, b, d_dm, spin_channel, scaling_factor, prec, d_out);

@@ -49,6 +49,10 @@ struct rjk_ctx {
     size_t ws_limit = size_t(4) << 30;
     void* ws = nullptr;       // grow-only workspace
     size_t ws_size = 0;
+    void* io = nullptr;       // grow-only staging of the host-pointer entry points (inputs, outputs)
+    size_t io_size = 0;
+    void* aux = nullptr;      // grow-only scratch of the density-only route (eigenpairs, statistics)
+    size_t aux_size = 0;
     void* solver = nullptr;   // rocblas_handle, created on first use (density-only K, SCF driver)
     void* scf_ws = nullptr;   // SCF driver buffers, grow-only, reused by every driver call
     size_t scf_ws_bytes = 0;
@@ -81,6 +85,14 @@ int solver_handle(rjk_ctx* ctx, _rocblas_handle** h);
 
 // workspace: returns a device buffer of at least `bytes` (grow-only)
 int ws_get(rjk_ctx* ctx, size_t bytes, void** out);
+// any grow-only device buffer of the context (*buf, *size): at least `bytes` in *out
+int buf_get(rjk_ctx* ctx, void** buf, size_t* size, size_t bytes, void** out);
+// J on a given workspace (at least j_ws_bytes); `timed` records phase events
+size_t j_ws_bytes(const rjk_ctx* ctx, int nspin, int prec);
+int j_run(rjk_ctx* ctx, const void* b, int prec, const double* dm_dev, int nspin, double scale, int full_dm,
+          double* vj_dev, void* ws, bool timed);
+// the B copy and J precision of a J call (checks spin_channel, B state and prec)
+int j_source(rjk_ctx* ctx, size_t spin_channel, int prec, const void** bptr, int* jprec);
 // device B storage: allocate the copies `mask` asks for; then refresh the FP32 copy and synchronise
 int alloc_b(rjk_ctx* ctx, unsigned mask);
 int finish_b(rjk_ctx* ctx);
@@ -90,7 +102,7 @@ struct PhaseScope {
     rjk_ctx* ctx;
     rjk_ctx::EvPair ev;
     bool on;
-    PhaseScope(rjk_ctx* c, int phase, int prec = 0, double work = 0.0);
+    PhaseScope(rjk_ctx* c, int phase, int prec = 0, double work = 0.0, bool allow = true);
     ~PhaseScope();
 };
 
@@ -153,5 +165,13 @@ hipError_t launch_to_fp32(hipStream_t st, const double* b, size_t n, float* out)
 hipError_t launch_symmetrize(hipStream_t st, const double* dm, size_t nbf, double* out);
 hipError_t launch_copy(hipStream_t st, void* dst, const void* src, size_t bytes);
 hipError_t launch_check_finite(hipStream_t st, const double* x, size_t n, int* flag);
+// density-only K on a general D: asymmetry statistics (2 doubles per block, at most
+// kAsymBlocks blocks), the column sets [D | I] and the packed upper triangle of G
+constexpr size_t kAsymBlocks = 256;
+unsigned asym_stats_blocks(size_t nbf);
+hipError_t launch_asym_stats(hipStream_t st, const double* dm, size_t nbf, double* out);
+hipError_t launch_prep_general(hipStream_t st, const double* dm, size_t nbf, size_t nk, size_t nrow, double* cx,
+                               double* cy);
+hipError_t launch_pack_upper(hipStream_t st, const double* g, size_t nbf, size_t ldg, double scale, double* vk);
 
 }  // namespace rjk

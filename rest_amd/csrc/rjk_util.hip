@@ -1,7 +1,10 @@
 // rjk_util.hip — layout and housekeeping kernels: synthetic B fill, packing
 // of full B columns (REST ri3fn) into the packed pair layout, padding of
 // compact packed columns to the device leading dimension, FP32 copies,
-// symmetrisation of D, a plain copy (counter calibration) and a finite check.
+// symmetrisation of D, a plain copy (counter calibration), a finite check and
+// the helpers of the density-only K on a general D.
+#include <algorithm>
+
 #include "rjk_internal.hpp"
 
 namespace rjk {
@@ -96,6 +99,60 @@ __global__ void k_check_finite(const double* __restrict__ x, size_t n, int* __re
     if (bad) *flag = 1;   // benign race: every writer stores 1
 }
 
+// Per-block maxima of |D(i,j) − D(j,i)| and |D(i,j)| over a column-major nbf×nbf D:
+// out[2·block] and out[2·block+1]; a non-finite entry makes both +inf.
+__global__ __launch_bounds__(256) void k_asym_stats(const double* __restrict__ dm, size_t nbf,
+                                                    double* __restrict__ out) {
+    const size_t n = nbf * nbf;
+    double a = 0.0, m = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = t % nbf, j = t / nbf;
+        const double x = dm[t], y = dm[i * nbf + j];
+        if (!isfinite(x)) { a = INFINITY; m = INFINITY; continue; }
+        a = fmax(a, fabs(x - y));
+        m = fmax(m, fabs(x));
+    }
+    __shared__ double ra[256], rm[256];
+    ra[threadIdx.x] = a;
+    rm[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            ra[threadIdx.x] = fmax(ra[threadIdx.x], ra[threadIdx.x + h]);
+            rm[threadIdx.x] = fmax(rm[threadIdx.x], rm[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = ra[0];
+        out[2 * blockIdx.x + 1] = rm[0];
+    }
+}
+
+// Column sets of the general density-only route, row-major [nrow][nk] like C':
+// cx[ν][c] = D(ν, c) (column c of the column-major D), cy[ν][c] = δ_νc, both zero
+// outside nbf × nbf.
+__global__ void k_prep_general(const double* __restrict__ dm, size_t nbf, size_t nk, size_t nrow,
+                               double* __restrict__ cx, double* __restrict__ cy) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrow * nk) return;
+    const size_t c = t % nk, nu = t / nk;
+    const bool in = nu < nbf && c < nbf;
+    cx[t] = in ? dm[nu + c * nbf] : 0.0;
+    cy[t] = (in && nu == c) ? 1.0 : 0.0;
+}
+
+// vk[p(i,j)] = scale · G(i, j) for i <= j, G column-major with leading dimension ldg.
+__global__ void k_pack_upper(const double* __restrict__ g, size_t nbf, size_t ldg, double scale,
+                             double* __restrict__ vk) {
+    const size_t npair = nbf * (nbf + 1) / 2;
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npair) return;
+    size_t i, j;
+    unpair(p, &i, &j);
+    vk[p] = scale * g[i + j * ldg];
+}
+
 unsigned grid_for(size_t n) {
     size_t g = (n + 255) / 256;
     if (g > 65536) g = 65536;
@@ -134,6 +191,26 @@ hipError_t launch_copy(hipStream_t st, void* dst, const void* src, size_t bytes)
     if (rem)
         hipLaunchKernelGGL(k_copy1, dim3(1), dim3(256), 0, st, (const unsigned char*)src + n16 * 16,
                            (unsigned char*)dst + n16 * 16, rem);
+    return hipGetLastError();
+}
+unsigned asym_stats_blocks(size_t nbf) {
+    size_t g = (nbf * nbf + 255) / 256;
+    return (unsigned)std::min<size_t>(std::max<size_t>(g, 1), kAsymBlocks);
+}
+hipError_t launch_asym_stats(hipStream_t st, const double* dm, size_t nbf, double* out) {
+    hipLaunchKernelGGL(k_asym_stats, dim3(asym_stats_blocks(nbf)), dim3(256), 0, st, dm, nbf, out);
+    return hipGetLastError();
+}
+hipError_t launch_prep_general(hipStream_t st, const double* dm, size_t nbf, size_t nk, size_t nrow, double* cx,
+                               double* cy) {
+    size_t n = nrow * nk;
+    hipLaunchKernelGGL(k_prep_general, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dm, nbf, nk, nrow, cx,
+                       cy);
+    return hipGetLastError();
+}
+hipError_t launch_pack_upper(hipStream_t st, const double* g, size_t nbf, size_t ldg, double scale, double* vk) {
+    size_t n = nbf * (nbf + 1) / 2;
+    hipLaunchKernelGGL(k_pack_upper, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, nbf, ldg, scale, vk);
     return hipGetLastError();
 }
 hipError_t launch_check_finite(hipStream_t st, const double* x, size_t n, int* flag) {

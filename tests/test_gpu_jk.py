@@ -64,6 +64,9 @@ def test_h2o_shape_golden(gold):
         assert maxabs(vk[0], gold["vk_rhf"]) <= TOL
         vkd = jk.vk_upper_with_rimatr_use_dm_only_sync(ri, [gold["dm"]], 1, -0.5)
         assert maxabs(vkd[0], gold["vk_rhf"]) <= TOL
+        # REST's density-only K reads a general D in full (src/scf_io/mod.rs:2730-2732)
+        vka = jk.vk_upper_with_rimatr_use_dm_only_sync(ri, [gold["dm_asym"]], 1, 1.0)
+        assert maxabs(vka[0], gold["vk_dm_asym"]) <= TOL
         ca, cb = gold["ca"], gold["cb"]
         cbp = np.zeros_like(ca)
         cbp[:, :cb.shape[1]] = cb
@@ -83,6 +86,31 @@ def test_h2o_shape_ri3fn_entry_points(gold):
         assert maxabs(vj[0], gold["vj_asym_ri3fn"]) <= TOL
         vk = jk.vk_upper_with_ri_v_use_dm_only_sync(ri, [gold["dm"]], 1, -0.5)
         assert maxabs(vk[0], gold["vk_rhf"]) <= TOL
+        vka = jk.vk_upper_with_ri_v_use_dm_only_sync(ri, [gold["dm_asym"]], 1, 1.0)
+        assert maxabs(vka[0], gold["vk_dm_asym"]) <= TOL
+
+
+def test_density_only_general_d_ragged():
+    """General (asymmetric) D through the density-only route: both spins, a ragged shape whose
+    nbf is no multiple of any tile, several auxiliary chunks (small workspace), against the oracle."""
+    nbf, naux = 77, 131
+    say(f"density-only general D nbf={nbf} naux={naux}")
+    rng = np.random.default_rng(81)
+    npair = nbf * (nbf + 1) // 2
+    b = rng.standard_normal((npair, naux)) / np.sqrt(naux)
+    da = rng.standard_normal((nbf, nbf))
+    db = rng.standard_normal((nbf, nbf))
+    db = db @ db.T / nbf
+    db[3, 5] += 0.25                                   # one asymmetric spin makes the call general
+    ref = orc.vk_upper_dm(b, True, [da, db], 2, -1.0)
+    with jk.DeviceRI(nbf, naux) as ri:
+        ri.set_rimatr(b)
+        ri.set_workspace_limit(6 << 20)                # forces several auxiliary chunks
+        got = ri.vk_dm([da, db], 2, -1.0)
+        assert maxabs(got, ref) <= TOL
+        # a requested low precision does not change the general route (FP64)
+        ri.set_rimatr(b, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        assert maxabs(ri.vk_dm([da, db], 2, -1.0, prec=jk.PREC_BF16X2), ref) <= TOL
 
 
 def test_device_synthetic_fill_equals_host_values(gold):
@@ -175,7 +203,7 @@ def test_density_only_route_indefinite_d():
 
 def test_density_only_device_route():
     """rjk_vk_upper_dm_dev: the density-only route on device memory equals the host entry point
-    bit for bit, matches the oracle, and symmetrises an asymmetric D where the host route rejects it."""
+    bit for bit and matches the oracle, for a symmetric D and for an asymmetric one (general route)."""
     import torch
     nbf, naux = 40, 90
     say(f"density-only device route nbf={nbf} naux={naux}")
@@ -191,17 +219,16 @@ def test_density_only_device_route():
         ri.set_rimatr(b)
         host = ri.vk_dm([d], 1, 0.5)
         out = torch.empty((1, npair), dtype=torch.float64, device=dev)
-        for dm, exact in ((d, True), (d + skew, False)):
+        host_skew = ri.vk_dm([d + skew], 1, 0.5)
+        for dm, want in ((d, host), (d + skew, host_skew)):
             dm_dev = torch.from_numpy(np.ascontiguousarray(dm.T)).to(dev)   # column-major
             torch.cuda.synchronize()
             ri.vk_dm_dev(dm_dev.data_ptr(), 1, 0.5, out.data_ptr())
             ri.synchronize()
-            got = out.cpu().numpy()
-            # the skewed D symmetrises to d up to the rounding of d + skew
-            assert np.array_equal(got, host) if exact else maxabs(got, host) <= 1e-13
+            assert np.array_equal(out.cpu().numpy(), want)
         assert maxabs(host, orc.vk_upper_dm(b, True, [d], 1, 0.5)) <= TOL
-        with pytest.raises(jk.RJKError):
-            ri.vk_dm([d + skew], 1, 0.5)
+        assert maxabs(host_skew, orc.vk_upper_dm(b, True, [d + skew], 1, 0.5)) <= TOL
+        assert maxabs(host_skew, host) > 1e-6          # the skew part is not dropped
         # two spins in one call (spin-major device buffers), FP64 and three BF16 planes
         d2 = rng.standard_normal((nbf, nbf))
         d2 = d2 @ d2.T / nbf
@@ -299,9 +326,10 @@ def test_error_paths():
         assert e.value.code == 1  # homo >= num_state
         asym = np.eye(nbf)
         asym[0, 1] = 1.0
+        assert np.isfinite(ri.vk_dm([asym], 1, 1.0)).all()   # general D: no error
         with pytest.raises(jk.RJKError) as e:
-            ri.vk_dm([asym], 1, 1.0)
-        assert e.value.code == 1
+            ri.vk_dm([bad], 1, 1.0)
+        assert e.value.code == 5
         # the context still works after errors
         assert np.isfinite(ri.vj([d], 1, 1.0)).all()
 
