@@ -132,6 +132,17 @@ int rjk_vj_upper_with_ri_v_sync(rjk_ctx* ctx, const double* dm, size_t spin_chan
  * ri3fn semantics (1) or the rimatr upper-triangle semantics (0). */
 int rjk_vj_upper_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel,
                      double scaling_factor, int prec, int full_dm, double* vj_dev);
+/* Allocation-free form of rjk_vj_upper_dev.  It has no counterpart in REST: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as rjk_vj_upper_dev, but the workspace is one
+ * fixed 128 MiB device array per GPU (held until the process ends) and the call
+ * allocates nothing.  A shape whose J workspace exceeds 128 MiB (about
+ * 8·spin_channel·npair·(1 + 4) bytes; nbf 1536 fits with two spins) returns
+ * RJK_ERR_ARG.  Every call on one GPU shares that array and leaves its last
+ * contents there: calls that overlap on one GPU race. */
+int rjk_vj_upper_dev_noalloc(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel,
+                             double scaling_factor, int prec, int full_dm, double* vj_dev);
 
 /* ---- K (exchange) ------------------------------------------------------- */
 

@@ -116,6 +116,9 @@ extern "C" int rjk_create(int device, size_t num_basis, size_t num_auxbas, rjk_c
         return fail(nullptr, RJK_ERR_HIP, "rjk_create: stream creation failed");
     }
     c->stream = c->own_stream;
+    // The allocation-free J entry point's scratch: its failure only disables that entry point.
+    if (scratch_init(c) != RJK_OK) c->scratch = nullptr;
+    c->err.clear();
     *out = c;
     return RJK_OK;
 }
@@ -345,7 +348,7 @@ JLayout j_layout(const rjk_ctx* ctx, int ns, int prec) {
 size_t j_ws_bytes(const rjk_ctx* ctx, int nspin, int prec) { return j_layout(ctx, nspin, prec).cv.total; }
 
 // Every workspace element a J call reads is written earlier in the same call, so the
-// workspace needs no clearing.
+// workspace needs no clearing (the allocation-free entry point relies on this).
 int j_run(rjk_ctx* ctx, const void* b, int prec, const double* dm_dev, int ns, double scale, int full_dm,
           double* vj_dev, void* ws, bool timed) {
     const JLayout L = j_layout(ctx, ns, prec);

@@ -53,6 +53,7 @@ struct rjk_ctx {
     size_t io_size = 0;
     void* aux = nullptr;      // grow-only scratch of the density-only route (eigenpairs, statistics)
     size_t aux_size = 0;
+    void* scratch = nullptr;  // this device's allocation-free J scratch (address from hipGetSymbolAddress)
     void* solver = nullptr;   // rocblas_handle, created on first use (density-only K, SCF driver)
     void* scf_ws = nullptr;   // SCF driver buffers, grow-only, reused by every driver call
     size_t scf_ws_bytes = 0;
@@ -87,6 +88,9 @@ int solver_handle(rjk_ctx* ctx, _rocblas_handle** h);
 int ws_get(rjk_ctx* ctx, size_t bytes, void** out);
 // any grow-only device buffer of the context (*buf, *size): at least `bytes` in *out
 int buf_get(rjk_ctx* ctx, void** buf, size_t* size, size_t bytes, void** out);
+// the allocation-free J scratch of ctx->device: looked up once per device (rjk_create)
+int scratch_init(rjk_ctx* ctx);
+size_t scratch_bytes();
 // J on a given workspace (at least j_ws_bytes); `timed` records phase events
 size_t j_ws_bytes(const rjk_ctx* ctx, int nspin, int prec);
 int j_run(rjk_ctx* ctx, const void* b, int prec, const double* dm_dev, int nspin, double scale, int full_dm,

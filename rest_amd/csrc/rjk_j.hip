@@ -13,6 +13,11 @@
 //
 // Both passes stream B once: the J build moves 2·|B| bytes of HBM traffic,
 // its roofline bound.  Spins share the B stream (both γ, both J at once).
+//
+// The allocation-free J entry point at the end of this file runs the same
+// kernels on a file-scope device array instead of the context's workspace.
+#include <mutex>
+
 #include "rjk_internal.hpp"
 
 namespace rjk {
@@ -306,3 +311,69 @@ hipError_t launch_jreduce(hipStream_t st, const double* jpart, size_t nsplit, in
 }
 
 }  // namespace rjk
+
+// ---------------------------------------------------------------------------
+// Allocation-free J.  Its workspace is `scratch`, one fixed device array per GPU
+// (held until the process ends).  Host code never passes `scratch` itself to a
+// kernel: the array's name is a host-side stand-in there, and the device address
+// comes from hipGetSymbolAddress, looked up once per device by scratch_init()
+// (called from rjk_create with that device current) and kept per device.
+// Calls on one GPU share the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace rjk {
+constexpr size_t kScratchBytes = size_t(128) << 20;   // (H2O)64's J fits with two spins
+constexpr int kScratchMaxDevices = 64;
+}  // namespace rjk
+
+static __device__ __align__(256) unsigned char scratch[rjk::kScratchBytes];
+
+namespace rjk {
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kScratchMaxDevices] = {};
+}  // namespace
+
+size_t scratch_bytes() { return kScratchBytes; }
+
+int scratch_init(rjk_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kScratchMaxDevices)
+        return fail(ctx, RJK_ERR_ARG, "no-alloc scratch: device index beyond the per-device table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return fail(ctx, RJK_ERR_HIP, std::string("no-alloc scratch: hipGetSymbolAddress failed: ") +
+                                              hipGetErrorString(e));
+        }
+        g_scratch_dev[ctx->device] = ptr;
+    }
+    ctx->scratch = g_scratch_dev[ctx->device];
+    return RJK_OK;
+}
+
+}  // namespace rjk
+
+extern "C" int rjk_vj_upper_dev_noalloc(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel,
+                                        double scaling_factor, int prec, int full_dm, double* vj_dev) {
+    using namespace rjk;
+    if (!ctx) return RJK_ERR_ARG;
+    if (!dm_dev || !vj_dev) return fail(ctx, RJK_ERR_ARG, "vj (no-alloc): NULL device pointer");
+    if (spin_channel != 1 && spin_channel != 2)
+        return fail(ctx, RJK_ERR_ARG, "spin_channel must be 1 or 2, got " + std::to_string(spin_channel));
+    // the size limit does not depend on which B copy the call ends up reading
+    const int ns = (int)spin_channel;
+    const size_t need = std::max(j_ws_bytes(ctx, ns, RJK_PREC_FP64), j_ws_bytes(ctx, ns, RJK_PREC_FP32));
+    if (need > kScratchBytes)
+        return fail(ctx, RJK_ERR_ARG, "vj (no-alloc): the J workspace of this shape, " + std::to_string(need) +
+                                          " bytes, exceeds the fixed scratch of " + std::to_string(kScratchBytes) +
+                                          " bytes; use rjk_vj_upper_dev");
+    const void* b = nullptr;
+    int jprec = prec;
+    int rc = j_source(ctx, spin_channel, prec, &b, &jprec);
+    if (rc) return rc;
+    if (!ctx->scratch) return fail(ctx, RJK_ERR_HIP, "vj (no-alloc): scratch address unavailable on this device");
+    RJK_HIP(ctx, hipSetDevice(ctx->device));
+    return j_run(ctx, b, jprec, dm_dev, ns, scaling_factor, full_dm, vj_dev, ctx->scratch, false);
+}

@@ -43,7 +43,7 @@ EXPORTS = [
     "rjk_create", "rjk_destroy", "rjk_last_error", "rjk_num_basis", "rjk_num_auxbas", "rjk_num_baspar",
     "rjk_ld_baspar", "rjk_set_stream", "rjk_synchronize", "rjk_set_workspace_limit", "rjk_set_rimatr",
     "rjk_set_rimatr_dev", "rjk_set_ri3fn", "rjk_fill_rimatr_synthetic", "rjk_rimatr_dev", "rjk_get_rimatr",
-    "rjk_vj_upper_with_rimatr_v_sync", "rjk_vj_upper_with_ri_v_sync", "rjk_vj_upper_dev",
+    "rjk_vj_upper_with_rimatr_v_sync", "rjk_vj_upper_with_ri_v_sync", "rjk_vj_upper_dev", "rjk_vj_upper_dev_noalloc",
     "rjk_vk_upper_with_ri_v_sync", "rjk_vk_upper_occ_dev", "rjk_vk_upper_with_rimatr_use_dm_only_sync",
     "rjk_vk_upper_with_ri_v_use_dm_only_sync", "rjk_vk_upper_dm_dev", "rjk_profile_enable", "rjk_profile_read", "rjk_profile_read_detail", "rjk_copy_dev",
     # include/rest_ri.h
@@ -108,6 +108,7 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rjk_vj_upper_with_rimatr_v_sync": (i, [vp, vp, sz, d, i, vp]),
         "rjk_vj_upper_with_ri_v_sync": (i, [vp, vp, sz, d, i, vp]),
         "rjk_vj_upper_dev": (i, [vp, vp, sz, d, i, i, vp]),
+        "rjk_vj_upper_dev_noalloc": (i, [vp, vp, sz, d, i, i, vp]),
         "rjk_vk_upper_with_ri_v_sync": (i, [vp, vp, sz, vp, vp, sz, d, i, vp]),
         "rjk_vk_upper_occ_dev": (i, [vp, vp, sz, vp, vp, sz, d, i, vp]),
         "rjk_vk_upper_with_rimatr_use_dm_only_sync": (i, [vp, vp, sz, d, i, vp]),
@@ -295,6 +296,13 @@ class DeviceRI:
                prec: int = PREC_FP64, full_dm: bool = False):
         self._check(self.lib.rjk_vj_upper_dev(self.h, C.c_void_p(dm_ptr), spin_channel, float(scaling_factor),
                                               prec, int(full_dm), C.c_void_p(out_ptr)))
+
+    def vj_dev_noalloc(self, dm_ptr: int, spin_channel: int, scaling_factor: float, out_ptr: int,
+                       prec: int = PREC_FP64, full_dm: bool = False):
+        """rjk_vj_upper_dev_noalloc: vj_dev on the library's fixed device scratch (no allocation)."""
+        self._check(self.lib.rjk_vj_upper_dev_noalloc(self.h, C.c_void_p(dm_ptr), spin_channel,
+                                                      float(scaling_factor), prec, int(full_dm),
+                                                      C.c_void_p(out_ptr)))
 
     def vk_occ_dev(self, eigv_ptr: int, num_state: int, homo: Sequence[int], occ_ptr: int, spin_channel: int,
                    scaling_factor: float, out_ptr: int, prec: int = PREC_FP64):
