@@ -9,6 +9,11 @@
  * before start_diis_cycle and Pulay DIIS on e = F D S − S D F afterwards
  * (prepare_next_input, :2947-3091; diis_solver, :3138-3188), until |ΔE|,
  * ‖ΔD‖_F/nbf and ‖Δε‖₂ meet the thresholds (check_scf_convergence, :1875-1951).
+ * As in REST, the first iteration is never taken as converged (the check
+ * returns early while num_iter < 2, :1877-1879), the loop stops after
+ * iteration max(max_scf_cycle, 2) (:1947), and after start_check_oscillation
+ * DIIS steps an energy rise switches to mixing with an earlier Fock matrix
+ * (:3002-3010, :3039-3048).
  *
  * All matrices stay in HBM; J/K use the context's resident B.  With
  * jk_prec = RJK_PREC_FP32 (or RJK_PREC_BF16X3 / RJK_PREC_BF16X2: K on the
@@ -60,6 +65,16 @@ typedef struct rscf_ctrl {
     double guess_err_max;    /* return before the first iteration when the guess Fock build's
                               * max|F D S − S D F| >= this (result.guess_rejected = 1); HUGE_VAL
                               * (default): never */
+    int start_check_oscillation; /* REST start_check_oscillation (20, src/ctrl_io/mod.rs:220): once more
+                              * than this many DIIS steps have recorded their energy, an energy rise
+                              * replaces the DIIS Fock matrix by (1 − mix_param)·H[i−1] + mix_param·H[i+1]
+                              * (src/scf_io/mod.rs:3002-3010, :3039-3048); energies of different J/K
+                              * precisions are not compared */
+    double inc_refresh;      /* incremental tail: when max|D − D_ref| > inc_refresh·max|D_ref|, the
+                              * FP64 build is a full one and becomes the new reference (1e-5) */
+    int inc_check;           /* 1: every incremental build is followed by a full FP64 build of the same
+                              * density, and max|F_inc − F_full| / max|F_full| goes to the log
+                              * (RSCF_LOG_INC_ERR); a test mode, 0 by default */
 } rscf_ctrl;
 
 /* What DIIS keeps at the precision switch.  RESTART: the stored low-precision
@@ -71,7 +86,12 @@ enum { RSCF_SWITCH_RESTART = 0, RSCF_SWITCH_SHIFT = 1 };
 
 /* Per-iteration record fields (iter_log rows of RSCF_LOG_FIELDS doubles). */
 enum { RSCF_LOG_E = 0, RSCF_LOG_DE, RSCF_LOG_DIIS_MAX, RSCF_LOG_DM, RSCF_LOG_EEV, RSCF_LOG_PREC,
-       RSCF_LOG_JK_MS, RSCF_LOG_DIAG_MS, RSCF_LOG_ITER_MS, RSCF_LOG_FIELDS };
+       RSCF_LOG_JK_MS, RSCF_LOG_DIAG_MS, RSCF_LOG_ITER_MS,
+       RSCF_LOG_DREF,      /* incremental build: max|D − D_ref| / max|D_ref|; else 0            */
+       RSCF_LOG_INC_ERR,   /* inc_check: max|F_inc − F_full| / max|F_full|; else 0              */
+       RSCF_LOG_OSC,       /* 1: the energy-rise mixing replaced the DIIS Fock matrix            */
+       RSCF_LOG_EXACT,     /* 1: the orbitals came from the eigensolver, 0: pseudo-diagonalised  */
+       RSCF_LOG_FIELDS };
 
 typedef struct rscf_result {
     double energy;           /* final total energy (Eh)                    */
@@ -85,6 +105,7 @@ typedef struct rscf_result {
                               * ctrl.guess_err_max, and no iteration ran */
     int incremental_iterations; /* FP64 iterations whose J/K were increments on the reference build */
     double guess_error;      /* max|F D S − S D F| of the guess build */
+    int reference_builds;    /* full FP64 reference builds of the incremental tail */
 } rscf_result;
 
 void rscf_default_ctrl(rscf_ctrl* ctrl);
@@ -93,7 +114,7 @@ void rscf_default_ctrl(rscf_ctrl* ctrl);
  * core Hamiltonian); enuc: nuclear repulsion; nelec: electron count (even);
  * d_guess: host nbf×nbf guess density or NULL for the core-Hamiltonian guess.
  * Outputs (each may be NULL): mo_coeff nbf×nbf, mo_energy nbf, dm nbf×nbf,
- * iter_log max_scf_cycle × RSCF_LOG_FIELDS (row-major). */
+ * iter_log max(max_scf_cycle, 2) × RSCF_LOG_FIELDS (row-major). */
 int rscf_rhf(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nelec, const double* d_guess,
              const rscf_ctrl* ctrl, rscf_result* result, double* mo_coeff, double* mo_energy, double* dm,
              double* iter_log);

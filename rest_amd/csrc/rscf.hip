@@ -22,6 +22,7 @@
 namespace {
 
 constexpr int kRedBlocks = 1024;
+constexpr double kGapMin = 0.05;   // HOMO-LUMO gap below which pseudo-diagonalisation is not used (Eh)
 
 double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -137,6 +138,23 @@ __global__ void k_inc_columns(const double* __restrict__ A, const double* __rest
         UV[r + i * n] = ca * p + cb * m;
         UV[r + (nocc + i) * n] = ca * p - cb * m;
     }
+}
+
+// partial[b] = max |x|  and  max |a − b|
+__global__ void k_maxabs(const double* __restrict__ x, size_t n, double* __restrict__ partial) {
+    double m = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[t]));
+    const double r = block_max(m);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+__global__ void k_maxabs_diff(const double* __restrict__ a, const double* __restrict__ b, size_t n,
+                              double* __restrict__ partial) {
+    double m = 0.0;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(a[t] - b[t]));
+    const double r = block_max(m);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
 // out = A − B
@@ -335,6 +353,9 @@ extern "C" void rscf_default_ctrl(rscf_ctrl* c) {
     c->pseudo_diag_err = 0.0;
     c->jk_prec_increment = -1;
     c->guess_err_max = HUGE_VAL;
+    c->start_check_oscillation = 20;
+    c->inc_refresh = 1e-5;
+    c->inc_check = 0;
 }
 
 namespace {
@@ -443,6 +464,17 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             vkref[s] = mem.get(ctx, npair, &rc);
         }
     }
+    // inc_check: a full FP64 J/K of the same density beside every incremental build
+    double *vjc = nullptr, *vkc[2] = {nullptr, nullptr};
+    if (inc_enabled && ctrl.inc_check) {
+        vjc = mem.get(ctx, npair, &rc);
+        for (int s = 0; s < nspin; ++s) vkc[s] = mem.get(ctx, npair, &rc);
+    }
+    // REST's prev_hamiltonian (src/scf_io/mod.rs:2851): the last two Fock matrices DIIS handed
+    // to the eigensolver, per spin, for the energy-rise mixing
+    double* hprev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    for (int k = 0; k < 2; ++k)
+        for (int s = 0; s < nspin; ++s) hprev[k][s] = mem.get(ctx, n2, &rc);
     if (rc) return rc;
     if (ctrl.switch_mode != RSCF_SWITCH_RESTART && ctrl.switch_mode != RSCF_SWITCH_SHIFT)
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": switch_mode must be RSCF_SWITCH_RESTART or _SHIFT");
@@ -566,7 +598,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                                (rocblas_int)n));
         hipLaunchKernelGGL(k_diag_copy, dim3((unsigned)((n + 255) / 256)), b256, 0, st, T2, n, W[s]);
         SCF_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_pseudo_rot, dim3(kRedBlocks), b256, 0, st, T2, n, nocc[s], 0.05, T1);
+        hipLaunchKernelGGL(k_pseudo_rot, dim3(kRedBlocks), b256, 0, st, T2, n, nocc[s], kGapMin, T1);
         SCF_HIP(hipGetLastError());
         SCF_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
                                (rocblas_int)n, &one, A[s], (rocblas_int)n, T1, (rocblas_int)n, &zero, Bw,
@@ -608,8 +640,12 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     // later FP64 builds are J_ref + J(D − D_ref) and K_ref + K(ΔD) with the increments at
     // jk_prec_increment, K(ΔD) through the [u | v] columns of k_inc_columns, so their rounding
     // error scales with |D − D_ref| (Almlöf, Faegri & Korsell, J. Comput. Chem. 3, 385 (1982)).
+    // A build is an increment only while max|D − D_ref| <= inc_refresh·max|D_ref|; beyond that
+    // (or after an exact diagonalisation dropped the reference) it is a full FP64 build and
+    // becomes the new reference.
     bool inc_armed = false, have_ref = false;
-    int n_inc = 0;
+    int n_inc = 0, n_refbuild = 0;
+    double dref_max = 0.0, last_dref = 0.0, last_inc_err = 0.0;
     auto fock = [&](double* const* Dsrc, bool occ_route, int prec, double* E) -> int {
         const double t0 = now();
         const double* Djk = Dsrc[0];
@@ -619,10 +655,21 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             Djk = Dt;
         }
         const bool inc_route = inc_armed && prec == RJK_PREC_FP64 && occ_route;
-        if (inc_route && have_ref) {
-            const int pinc = ctrl.jk_prec_increment;
+        bool use_inc = inc_route && have_ref;
+        last_dref = 0.0;
+        last_inc_err = 0.0;
+        if (use_inc) {
             hipLaunchKernelGGL(k_sub, dim3(kRedBlocks), b256, 0, st, Djk, Dref, n2, Dinc);
             SCF_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_maxabs, dim3(kRedBlocks), b256, 0, st, Dinc, n2, partial);
+            SCF_HIP(hipGetLastError());
+            double dd = 0.0;
+            SCF_RC(reduce_max(&dd));
+            last_dref = dref_max > 0.0 ? dd / dref_max : HUGE_VAL;
+            if (!(last_dref <= ctrl.inc_refresh)) use_inc = false;
+        }
+        if (use_inc) {
+            const int pinc = ctrl.jk_prec_increment;
             SCF_RC(rjk_vj_upper_dev(ctx, Dinc, 1, 1.0, pinc, 0, vj));
             hipLaunchKernelGGL(k_axpy_inplace, dim3(kRedBlocks), b256, 0, st, vj, vjref, 1.0, npair);
             SCF_HIP(hipGetLastError());
@@ -655,6 +702,10 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                 }
             }
             if (inc_route) {   // this FP64 build is the reference of the increments that follow
+                hipLaunchKernelGGL(k_maxabs, dim3(kRedBlocks), b256, 0, st, Djk, n2, partial);
+                SCF_HIP(hipGetLastError());
+                SCF_RC(reduce_max(&dref_max));
+                ++n_refbuild;
                 SCF_HIP(hipMemcpyAsync(Dref, Djk, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
                 SCF_HIP(hipMemcpyAsync(vjref, vj, npair * sizeof(double), hipMemcpyDeviceToDevice, st));
                 for (int s = 0; s < nspin; ++s) {
@@ -676,6 +727,28 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             e += v;
         }
         *E = enuc + 0.5 * e;
+        if (use_inc && ctrl.inc_check) {   // test mode: the full FP64 build of the same density
+            SCF_RC(rjk_vj_upper_dev(ctx, Djk, 1, 1.0, RJK_PREC_FP64, 0, vjc));
+            for (int s = 0; s < nspin; ++s) {
+                if (nocc[s] == 0) {
+                    hipLaunchKernelGGL(k_fill, dim3((unsigned)((npair + 255) / 256)), b256, 0, st, vkc[s], npair, 0.0);
+                    SCF_HIP(hipGetLastError());
+                } else {
+                    const size_t homo[2] = {nocc[s] - 1, 0};
+                    SCF_RC(rjk_vk_upper_occ_dev(ctx, A[s], n, homo, occd[s], 1, sk, RJK_PREC_FP64, vkc[s]));
+                }
+                hipLaunchKernelGGL(k_fock, g2, b256, 0, st, H, vjc, vkc[s], n, 1.0, T1);
+                SCF_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_maxabs_diff, dim3(kRedBlocks), b256, 0, st, F[s], T1, n2, partial);
+                SCF_HIP(hipGetLastError());
+                double d = 0.0, f = 0.0;
+                SCF_RC(reduce_max(&d));
+                hipLaunchKernelGGL(k_maxabs, dim3(kRedBlocks), b256, 0, st, T1, n2, partial);
+                SCF_HIP(hipGetLastError());
+                SCF_RC(reduce_max(&f));
+                last_inc_err = std::max(last_inc_err, f > 0.0 ? d / f : d);
+            }
+        }
         t_jk += now() - t0;
         return RJK_OK;
     };
@@ -777,11 +850,18 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     bool shift_pending = false;   // first FP64 build after the switch: shift the stored vectors
     std::vector<double> Bmat((size_t)nd * nd, 0.0);
     bool converged = false;
+    // REST's energy_records / prev_hamiltonian (src/scf_io/mod.rs:2850-2851, :3002-3057):
+    // energies recorded by the DIIS steps, and a list of at most two slots of Fock matrices
+    // that starts as one empty entry (-1 here)
+    int n_erec = 0, prec_erec = -1;
+    double E_erec = 0.0;
+    int hl[2] = {-1, -1}, hl_len = 1;
     if (ctrl.print_level > 0) std::printf("rscf: guess energy %.10f\n", E);
 
     while (true) {
         const double t_it = now();
         double diis_max = 0.0;
+        bool osc = false;
         const double jk0 = t_jk, dg0 = t_diag;
         const int prec_used = prec;
         auto linear_mix = [&]() -> int {   // n1[in] = n0[in] + α (n0[out] − n0[in])
@@ -805,6 +885,12 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                     SCF_HIP(hipMemcpyAsync(Fl[s], F[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
             }
             SCF_RC(fock(D, true, prec, &E));
+            // energy record and oscillation flag (REST :3002-3010); energies of different J/K
+            // precisions are not compared
+            ++n_erec;
+            osc = n_erec >= 2 && n_erec > ctrl.start_check_oscillation && prec_used == prec_erec && E - E_erec > 0.0;
+            E_erec = E;
+            prec_erec = prec_used;
             const double t0 = now();
             if (shift_pending && nvec > 0) {
                 // δ_s = F_s(FP64) − F_s(low) at the current density.  The low-precision
@@ -891,6 +977,28 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                     hipLaunchKernelGGL(k_lincomb, dim3(kRedBlocks), b256, 0, st, cf, nvec, n2, Fx[s]);
                     SCF_HIP(hipGetLastError());
                 }
+                // energy rise: (1 − α)·H[DIIS, i−1] + α·H[DIIS, i+1], H[DIIS, i−1] being the front of
+                // prev_hamiltonian; its empty first entry leaves the un-extrapolated F (REST :3039-3048)
+                if (osc) {
+                    for (int s = 0; s < nspin; ++s) {
+                        if (hl[0] >= 0)
+                            hipLaunchKernelGGL(k_mix, g2, b256, 0, st, hprev[hl[0]][s], Fx[s], ctrl.mix_param, n2,
+                                               Fx[s]);
+                        else
+                            SCF_HIP(hipMemcpyAsync(Fx[s], F[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+                        SCF_HIP(hipGetLastError());
+                    }
+                }
+                // prev_hamiltonian keeps the last two (REST :3056-3057)
+                if (hl_len == 2) {
+                    hl[0] = hl[1];
+                    hl_len = 1;
+                }
+                const int slot_h = hl[0] == 0 ? 1 : 0;
+                for (int s = 0; s < nspin; ++s)
+                    SCF_HIP(hipMemcpyAsync(hprev[slot_h][s], Fx[s], n2 * sizeof(double), hipMemcpyDeviceToDevice,
+                                           st));
+                hl[hl_len++] = slot_h;
             } else {   // DIIS failed: linear mixing now, DIIS again 8 cycles later (REST :3061-3078)
                 SCF_RC(linear_mix());
                 start_diis = num_iter + 8;
@@ -928,8 +1036,18 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         // below pseudo_diag_err rotate the previous orbitals instead
         // (after an incremental reference every iteration does, so that the occupied orbitals
         // stay aligned with C_ref)
-        const bool pseudo = have_ref || (prec_used != RJK_PREC_FP64 && num_iter >= start_diis && diis_max > 0.0 &&
-                                         diis_max < ctrl.pseudo_diag_err);
+        // The first-order rotation needs a HOMO-LUMO gap: below kGapMin (the denominator clamp of
+        // k_pseudo_rot) the eigensolver runs, and an incremental reference is dropped, since an
+        // exact diagonalisation may rotate the occupied orbitals away from C_ref.
+        bool gap_ok = true;
+        for (int s = 0; s < nspin; ++s)
+            if (nocc[s] > 0 && nocc[s] < n && eps[s][nocc[s]] - eps[s][nocc[s] - 1] < kGapMin) gap_ok = false;
+        bool pseudo = have_ref || (prec_used != RJK_PREC_FP64 && num_iter >= start_diis && diis_max > 0.0 &&
+                                   diis_max < ctrl.pseudo_diag_err);
+        if (pseudo && !gap_ok) {
+            pseudo = false;
+            have_ref = false;
+        }
         last_pseudo = pseudo;
         double dm_err[2] = {0.0, 0.0}, eev = 0.0;
         for (int s = 0; s < nspin; ++s) {
@@ -948,9 +1066,12 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         const double dm_max = std::max(dm_err[0], dm_err[1]);
         const double dE = E - E_prev;
         const bool both_fp64 = prec_E == RJK_PREC_FP64 && prec_prev == RJK_PREC_FP64;
-        converged = both_fp64 && std::fabs(dE) <= ctrl.scf_acc_etot && dm_max <= ctrl.scf_acc_rho &&
+        // REST checks nothing in the first iteration (num_iter < 2, :1877-1879) and stops at
+        // num_iter >= max_scf_cycle (:1947)
+        converged = num_iter >= 2 && both_fp64 && std::fabs(dE) <= ctrl.scf_acc_etot && dm_max <= ctrl.scf_acc_rho &&
                     eev <= ctrl.scf_acc_eev;
-        if (iter_log && num_iter - 1 < ctrl.max_scf_cycle) {
+        const bool at_limit = num_iter >= 2 && num_iter >= ctrl.max_scf_cycle;
+        if (iter_log && num_iter - 1 < std::max(ctrl.max_scf_cycle, 2)) {
             double* row = iter_log + (size_t)(num_iter - 1) * RSCF_LOG_FIELDS;
             row[RSCF_LOG_E] = E;
             row[RSCF_LOG_DE] = dE;
@@ -961,6 +1082,10 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
             row[RSCF_LOG_JK_MS] = (t_jk - jk0) * 1e3;
             row[RSCF_LOG_DIAG_MS] = (t_diag - dg0) * 1e3;
             row[RSCF_LOG_ITER_MS] = (now() - t_it) * 1e3;
+            row[RSCF_LOG_DREF] = prec_used == RJK_PREC_FP64 ? last_dref : 0.0;
+            row[RSCF_LOG_INC_ERR] = prec_used == RJK_PREC_FP64 ? last_inc_err : 0.0;
+            row[RSCF_LOG_OSC] = osc ? 1.0 : 0.0;
+            row[RSCF_LOG_EXACT] = pseudo ? 0.0 : 1.0;
         }
         if (ctrl.print_level > 0)
             std::printf("rscf: iter %3d  E %.10f  dE %.3e  diis %.3e  dD %.3e  eev %.3e  prec %s\n", num_iter, E, dE,
@@ -970,7 +1095,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         prec_prev = prec_E;
         for (int s = 0; s < nspin; ++s) eps_prev[s] = eps[s];
         ++num_iter;
-        if (converged || num_iter > ctrl.max_scf_cycle) break;
+        if (converged || at_limit) break;
     }
     if (last_pseudo && (mo_coeff || mo_energy || dm_out)) {   // canonical orbitals for the outputs
         for (int s = 0; s < nspin; ++s) {
@@ -992,6 +1117,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         result->guess_error = guess_err;
         result->guess_rejected = 0;
         result->incremental_iterations = n_inc;
+        result->reference_builds = n_refbuild;
     }
     for (int s = 0; s < nspin; ++s) {
         if (mo_coeff) SCF_HIP(hipMemcpy(mo_coeff + (size_t)s * n2, A[s], n2 * sizeof(double), hipMemcpyDeviceToHost));

@@ -14,7 +14,8 @@ import numpy as np
 
 from . import jk
 
-LOG_FIELDS = ("energy", "dE", "diis_max", "dm_err", "eev_err", "prec", "jk_ms", "diag_ms", "iter_ms")
+LOG_FIELDS = ("energy", "dE", "diis_max", "dm_err", "eev_err", "prec", "jk_ms", "diag_ms", "iter_ms", "dref",
+              "inc_err", "osc", "exact")
 
 
 class Ctrl(C.Structure):
@@ -23,7 +24,8 @@ class Ctrl(C.Structure):
                 ("scf_acc_etot", C.c_double), ("jk_prec", C.c_int), ("switch_diis_err", C.c_double),
                 ("print_level", C.c_int), ("switch_mode", C.c_int), ("jk_prec_start", C.c_int),
                 ("start_switch_err", C.c_double), ("guess_energy_max", C.c_double), ("pseudo_diag_err", C.c_double),
-                ("jk_prec_increment", C.c_int), ("guess_err_max", C.c_double)]
+                ("jk_prec_increment", C.c_int), ("guess_err_max", C.c_double),
+                ("start_check_oscillation", C.c_int), ("inc_refresh", C.c_double), ("inc_check", C.c_int)]
 
 
 SWITCH_RESTART, SWITCH_SHIFT = 0, 1
@@ -34,7 +36,7 @@ class Result(C.Structure):
                 ("fp64_iterations", C.c_int), ("wall_seconds", C.c_double), ("jk_seconds", C.c_double),
                 ("diag_seconds", C.c_double), ("diis_seconds", C.c_double), ("guess_energy", C.c_double),
                 ("guess_rejected", C.c_int), ("incremental_iterations", C.c_int),
-                ("guess_error", C.c_double)]
+                ("guess_error", C.c_double), ("reference_builds", C.c_int)]
 
 
 def _bind(lib):
@@ -103,6 +105,7 @@ class ScfOutcome:
     guess_rejected: bool = False    # the guess failed ctrl.guess_energy_max or ctrl.guess_err_max: no iteration ran
     incremental_iterations: int = 0  # FP64 iterations built as increments on the reference build
     guess_error: float = None       # max|F D S - S D F| of the guess build
+    reference_builds: int = 0       # full FP64 reference builds of the incremental tail
 
 
 def _colmajor(M):
@@ -135,7 +138,7 @@ def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals, c_guess=None) -> Scf
     C_ = np.empty((ns, n, n)) if want_orbitals else None
     eps = np.empty((ns, n))
     dm = np.empty((ns, n, n)) if want_orbitals else None
-    log = np.zeros((ctrl.max_scf_cycle, len(LOG_FIELDS)))
+    log = np.zeros((max(ctrl.max_scf_cycle, 2), len(LOG_FIELDS)))   # REST runs at least two iterations
     ptr = lambda a: None if a is None else a.ctypes.data
     cg = None if c_guess is None else _colmajor(c_guess)
     if ns == 1 and cg is not None:
@@ -155,7 +158,7 @@ def _run(ri, S, H, enuc, nel, d_guess, ctrl, want_orbitals, c_guess=None) -> Scf
     return ScfOutcome(res.energy, bool(res.converged), res.iterations, res.fp64_iterations, res.wall_seconds,
                       res.jk_seconds, res.diag_seconds, res.diis_seconds, eps[0] if ns == 1 else eps, tr(C_, False), tr(dm, True),
                       rows, res.guess_energy, bool(res.guess_rejected), res.incremental_iterations,
-                      res.guess_error)
+                      res.guess_error, res.reference_builds)
 
 
 def rhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nelec: int,

@@ -113,6 +113,31 @@ def test_sad_pentacene_reproduces_rest_guess_and_energy():
     assert abs(np.sum((da + db) * S) - m.nelectron) < 1e-9
 
 
+REST_SAD_UNREPRODUCIBLE = (
+    "REST's pentacene log (examples/PEN/RIHF/rest.out:47-85) was written by an older REST: it prints "
+    "'Initial guess HF energy', a string the source no longer holds (src/initial_guess/mod.rs:57 prints "
+    "'Initial guess energy').  Its guess is not reproducible from the source: E(D0) over every orientation "
+    "of carbon's two occupied 2p orbitals spans [-839.793, -831.211] Eh; the basis-aligned choices LAPACK makes "
+    "for exactly block-diagonal atomic matrices give -839.79073897 (this build), -832.750 and -831.216; no "
+    "orientation gives both REST's E(D0) -839.49686125 and its first-Fock energy -819.4804213896 (best "
+    "compromise -838.077 / -830.681), and REST's next energies (-612.14, -435.68 Eh) lie hundreds of Eh above "
+    "any density of this family.")
+
+
+@pytest.mark.skip(reason=REST_SAD_UNREPRODUCIBLE)
+def test_sad_pentacene_matches_rest_trace():
+    """E(D0), the first Fock energy and the per-iteration energies of REST's pentacene log."""
+    trace = json.loads((GOLD / "pentacene_rest_trace.json").read_text())
+    lines = (GOLD / "pentacene.xyz").read_text().splitlines()[2:]
+    sym = [l.split()[0] for l in lines]
+    xyz = np.array([[float(x) for x in l.split()[1:]] for l in lines])
+    m = ri.Molecule.from_angstrom(sym, xyz, basis="STO-3G", auxbasis="def2-SV(P)-JKFIT")
+    out, t = rs.run_molecule(m, guess="sad")
+    assert abs(t["e_guess_d0"] - trace["initial_guess_hf_energy"]) <= 1e-7
+    e_log = [row["energy"] for row in out.log]
+    np.testing.assert_allclose(e_log, trace["iteration_energies"], rtol=0, atol=1e-7)
+
+
 def test_sad_water_cluster_beats_core_guess():
     say("(H2O)4 SAD vs core-Hamiltonian guess")
     m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))

@@ -1,0 +1,138 @@
+"""GPU tests at the (H2O)64 shape (nbf 1536, naux 7232, nocc 320), real B from this
+build's integral engine, through the C ABI.
+
+  - (H2O)64 seed 2 (the headline workload): the low-precision J/K errors
+    against the FP64 J/K of the same converged orbitals; plain FP64 from SAD
+    against the mixed ladder from the learned guess, both to 1e-8 Eh, under
+    both acceptance rules; the incremental FP64 tail checked build by build
+    against full FP64 builds of the same density (inc_check).
+  - two held-out (H2O)64 clusters (seeds 100 and 101, never used in
+    training): the learned guess is kept by the energy rule, saves
+    iterations over SAD in the FP64 SCF, and ends at SAD's energy.
+
+Bars: |ΔE| <= 1e-9 Eh between SCFs converged to REST's defaults; K relative
+max error against the FP64 K of the converged orbitals <= 5e-5 (BF16X2),
+<= 1e-5 (BF16X3, FP32), J on the FP32 copy <= 5e-6; an incremental FP64 Fock
+matrix within 1e-10 (relative max) of the full FP64 one.
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from rest_amd import guess as gs
+from rest_amd import jk, ri
+from rest_amd import learned as lg
+from rest_amd import scf as rs
+from rest_amd import workload as wl
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parents[1]
+MODEL = ROOT / "rest_amd" / "models" / "learned_guess_h2o.npz"
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def read_xyz(path):
+    lines = path.read_text().splitlines()[2:]
+    return np.array([[float(x) for x in l.split()[1:]] for l in lines])
+
+
+def plain_scf(mol, r, S, H):
+    """FP64 from SAD: REST's guess step, then the SCF from those orbitals."""
+    da, db = gs.sad_density(mol)
+    _, _, c1, _ = rs.guess_step(r, S, H, mol.nuclear_repulsion(), (mol.nelectron,), da + db, jk.PREC_FP64)
+    return rs.rhf(r, S, H, mol.nuclear_repulsion(), mol.nelectron, None, rs.default_ctrl(), c_guess=c1)
+
+
+@pytest.fixture(scope="module")
+def net():
+    n = lg.Net.load(MODEL)
+    yield n
+    n.close()
+
+
+@pytest.fixture(scope="module")
+def h2o64():
+    xyz = read_xyz(ROOT / "tests" / "golden" / "h2o64_seed2.xyz")
+    np.testing.assert_allclose(xyz, wl.water_cluster(2, 64), atol=1e-9)
+    mol = ri.Molecule.water_cluster(xyz)
+    say(f"(H2O)64 seed 2: B build nbf={mol.nbf} naux={mol.naux}, FP64 + FP32 + BF16 planes")
+    S, T, V = mol.int1e()
+    r = jk.DeviceRI(mol.nbf, mol.naux)
+    mol.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+    state = dict(mol=mol, r=r, S=S, H=T + V)
+    yield state
+    r.close()
+
+
+def test_h2o64_plain_vs_mixed_learned(h2o64, net):
+    mol, r, S, H = h2o64["mol"], h2o64["r"], h2o64["S"], h2o64["H"]
+    say("(H2O)64 plain FP64 from SAD")
+    plain = plain_scf(mol, r, S, H)
+    h2o64["plain"] = plain
+    say(f"  E {plain.energy:.10f} in {plain.iterations} iterations")
+    assert plain.converged
+    for guard in ("error", "energy"):
+        say(f"(H2O)64 mixed ladder from the learned guess, guard {guard}, increments checked")
+        g = lg.learned_scf(net, mol, r, S, H, rs.mixed_ctrl(inc_check=1), guard=guard)
+        o = g.outcome
+        precs = [int(x["prec"]) for x in o.log]
+        inc = [(x["dref"], x["inc_err"]) for x in o.log if x["inc_err"] > 0]
+        say(f"  kept {g.kind}, E {o.energy:.10f} (dE {o.energy - plain.energy:.1e}) in {o.iterations} iterations, "
+            f"precisions {precs}, {o.incremental_iterations} increments, {o.reference_builds} reference builds, "
+            f"(dref, F error) {inc}")
+        assert g.kind == "learned" and o.converged
+        assert abs(o.energy - plain.energy) <= 1e-9
+        assert precs[-1] == jk.PREC_FP64 and o.incremental_iterations >= 1
+        assert all(err <= 1e-10 for _, err in inc)
+
+
+def test_h2o64_low_precision_jk_errors(h2o64):
+    mol, r = h2o64["mol"], h2o64["r"]
+    plain = h2o64.get("plain")
+    if plain is None:
+        plain = h2o64["plain"] = plain_scf(mol, r, h2o64["S"], h2o64["H"])
+    nocc = mol.nelectron // 2
+    C = plain.mo_coeff
+    occ = np.zeros(mol.nbf)
+    occ[:nocc] = 2.0
+    D = plain.dm
+    say("(H2O)64 K at FP64, BF16X2, BF16X3 and J at FP64, FP32 of the converged orbitals")
+    k64 = r.vk_occ([C], [nocc - 1], [occ], 1, -0.5)[0]
+    scale = np.abs(k64).max()
+    errs = {}
+    for name, prec in (("bf16x2", jk.PREC_BF16X2), ("bf16x3", jk.PREC_BF16X3), ("fp32", jk.PREC_FP32)):
+        k = r.vk_occ([C], [nocc - 1], [occ], 1, -0.5, prec=prec)[0]
+        errs["k_" + name] = float(np.abs(k - k64).max() / scale)
+    j64 = r.vj([D], 1, 1.0)[0]
+    j32 = r.vj([D], 1, 1.0, prec=jk.PREC_FP32)[0]
+    errs["j_fp32"] = float(np.abs(j32 - j64).max() / np.abs(j64).max())
+    say(f"  relative max errors (real B, converged orbitals) {errs}")
+    # real B spans many magnitudes: two planes keep about 16 significant bits per element
+    assert errs["k_bf16x2"] <= 5e-5 and errs["k_bf16x3"] <= 1e-5 and errs["k_fp32"] <= 1e-5
+    assert errs["k_bf16x3"] < errs["k_bf16x2"]
+    assert errs["j_fp32"] <= 5e-6
+
+
+@pytest.mark.parametrize("seed", [100, 101])
+def test_learned_guess_gain_on_held_out_h2o64(seed, net):
+    mol = ri.Molecule.water_cluster(wl.water_cluster(seed, 64))
+    say(f"held-out (H2O)64 seed {seed}: FP64 from SAD vs FP64 from the learned guess (energy rule)")
+    S, T, V = mol.int1e()
+    H = T + V
+    with jk.DeviceRI(mol.nbf, mol.naux) as r:
+        mol.build_rimatr(r, 1e-8, jk.STORE_FP64)
+        sad = plain_scf(mol, r, S, H)
+        g = lg.learned_scf(net, mol, r, S, H, rs.default_ctrl(), guard="energy")
+    o = g.outcome
+    say(f"  SAD {sad.energy:.10f} ({sad.iterations} it)  learned {o.energy:.10f} ({o.iterations} it), "
+        f"kept {g.kind}, E(D_pred) {g.e_learned:.6f} vs SAD E(D1) {g.e_sad:.6f}")
+    assert sad.converged and o.converged and g.kind == "learned"
+    assert g.e_learned < g.e_sad
+    assert o.iterations < sad.iterations
+    assert abs(o.energy - sad.energy) <= 1e-9

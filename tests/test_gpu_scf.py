@@ -210,7 +210,8 @@ def test_incremental_fp64_builds_after_the_switch(nspin, pinc, mode):
     say(f"  E {om.energy:.12f} vs {o64.energy:.12f} (dE {om.energy - o64.energy:.2e}); iterations {om.iterations}, "
         f"fp64 {om.fp64_iterations}, incremental {om.incremental_iterations}")
     assert om.converged and abs(om.energy - o64.energy) <= 1e-9
-    assert om.incremental_iterations >= 1 and om.incremental_iterations == om.fp64_iterations - 1
+    assert om.incremental_iterations >= 1
+    assert om.incremental_iterations + om.reference_builds == om.fp64_iterations
     assert precs[-1] == jk.PREC_FP64
     # canonical outputs: C diagonalises F(D_out) and ε matches the FP64 run's
     cs = [om.mo_coeff] if nspin == 1 else list(om.mo_coeff)
@@ -219,6 +220,123 @@ def test_incremental_fp64_builds_after_the_switch(nspin, pinc, mode):
     for c, e, e_ref in zip(cs, es, e64):
         assert np.allclose(c.T @ S @ c, np.eye(len(e)), atol=1e-10)
         assert np.max(np.abs(e - e_ref)) < 1e-5
+
+
+def test_incremental_tail_after_an_early_switch_refreshes_its_reference():
+    """jk_prec FP64 with a BF16X2 start switches to FP64 at max|e| < start_switch_err (1e-3), far from
+    convergence, so the density leaves the first reference; a build whose max|D − D_ref| exceeds
+    inc_refresh·max|D_ref| becomes a new reference.  inc_check compares every increment with the
+    full FP64 Fock matrix of the same density."""
+    say("(H2O)4: FP64 after a BF16X2 start, BF16X2 increments, early switch, increments checked")
+    m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
+    S, T, V = m.int1e()
+    H = T + V
+    enuc = m.nuclear_repulsion()
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        o64 = rs.rhf(r, S, H, enuc, m.nelectron, None, tight())
+        ctrl = tight()
+        ctrl.jk_prec, ctrl.jk_prec_start, ctrl.jk_prec_increment = jk.PREC_FP64, jk.PREC_BF16X2, jk.PREC_BF16X2
+        ctrl.inc_check = 1
+        om = rs.rhf(r, S, H, enuc, m.nelectron, None, ctrl)
+    inc = [(x["dref"], x["inc_err"]) for x in om.log if x["inc_err"] > 0]
+    say(f"  E {om.energy:.12f} vs {o64.energy:.12f} (dE {om.energy - o64.energy:.2e}); {om.iterations} it, "
+        f"{om.incremental_iterations} increments, {om.reference_builds} references; (dref, F error) {inc}")
+    assert om.converged and abs(om.energy - o64.energy) <= 1e-9
+    assert om.reference_builds >= 2 and om.incremental_iterations >= 1
+    assert all(d <= 1e-5 for d, _ in inc) and all(e <= 1e-10 for _, e in inc)
+
+
+def test_energy_rise_mixing_on_the_device():
+    """REST's energy-rise mixing (src/scf_io/mod.rs:3002-3010, :3039-3057) in the device loop, on
+    pentacene from the core guess, whose energy rises early (the oracle loop is pinned the same
+    way on the CPU, tests/test_scf_oracle.py; the two paths are not compared step by step because
+    this start is chaotic: rounding-level differences grow to 1e-2 Eh within a few iterations):
+      - mixing flags exactly at the energy rises after start_check_oscillation records;
+      - mix_param 0: the mixed Fock matrix is H[i−1], so the energy after a mixing step k equals
+        the energy after step k−2, bit for bit;
+      - mix_param 1: the path with the check on equals the path with it off, bit for bit;
+      - start_check_oscillation 0: a rise at the second step meets the empty first entry, which
+        leaves the Fock matrix of the density (its energy path equals plain Roothaan there)."""
+    m = pentacene()
+    S, T, V = m.int1e()
+    H = T + V
+    enuc = m.nuclear_repulsion()
+    run = lambda r, **kw: rs.rhf(r, S, H, enuc, m.nelectron, None, rs.default_ctrl(max_scf_cycle=12, **kw))
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r)
+        say("pentacene core guess, check after 2 records, mix_param 0.6 / 0 / 1, and the check off")
+        o6 = run(r, start_check_oscillation=2)
+        o0 = run(r, start_check_oscillation=2, mix_param=0.0)
+        o1 = run(r, start_check_oscillation=2, mix_param=1.0)
+        off1 = run(r, start_check_oscillation=1000, mix_param=1.0)
+        z = run(r, start_check_oscillation=0)
+    for name, o in (("0.6", o6), ("0", o0)):
+        e = [x["energy"] for x in o.log]
+        osc = [bool(x["osc"]) for x in o.log]
+        say(f"  mix {name}: mixing at {[k + 1 for k, f in enumerate(osc) if f]}, energies "
+            + " ".join(f"{v:.6f}" for v in e))
+        # the k-th row holds the energy recorded at DIIS step k (k >= 1); REST's rule with 2
+        want = [k >= 2 and e[k] > e[k - 1] for k in range(len(e))]
+        assert osc == want and any(osc)
+    e0 = [x["energy"] for x in o0.log]
+    steps = [k for k, x in enumerate(o0.log) if x["osc"] and k >= 2 and k + 1 < len(e0)]
+    assert steps
+    for k in steps:   # row k+1 is the energy of the density made at step k (rows 0-based)
+        assert e0[k + 1] == e0[k - 1], k
+    assert [x["energy"] for x in o1.log] == [x["energy"] for x in off1.log]
+    assert any(x["osc"] for x in o1.log)
+    zo = [bool(x["osc"]) for x in z.log]
+    assert zo[1] and not zo[0]
+
+
+def test_diis_failure_on_a_one_function_atom():
+    """One basis function (H, STO-3G): F D S − S D F is zero, the DIIS matrix is singular, and the
+    driver falls back to linear mixing (REST :3061-3078, as in REST's own pentacene log for H);
+    the energy is the closed form H/S after REST's two iterations."""
+    say("H atom STO-3G UHF: DIIS failure path")
+    m = ri.Molecule(["H"], np.zeros((1, 3)), "STO-3G", "def2-SV(P)-JKFIT")
+    S, T, V = m.int1e()
+    H = T + V
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r)
+        out = rs.uhf(r, S, H, 0.0, 1, 0, None, rs.default_ctrl())
+    say(f"  E {out.energy:.14f} closed form {H[0, 0] / S[0, 0]:.14f}, {out.iterations} iterations")
+    assert out.converged and out.iterations == 2
+    assert abs(out.energy - H[0, 0] / S[0, 0]) < 1e-12
+
+
+def test_max_scf_cycle_one_still_runs_two_iterations():
+    say("(H2O)2 with max_scf_cycle = 1: REST's check starts at the second iteration")
+    m = ri.Molecule.water_cluster(wl.water_cluster(3, 2))
+    out, _ = rs.run_molecule(m, ctrl=rs.default_ctrl(max_scf_cycle=1))
+    assert out.iterations == 2 and not out.converged and len(out.log) == 2
+
+
+def test_single_water_and_h2o16_fp64_vs_oracle():
+    """The single-water workload (O at the origin, H at (±0.7572, 0.5865, 0) Å) and (H2O)16 seed 1 with
+    real B, FP64 from SAD at tight thresholds: against the oracle loop on the identical B (<= 1e-9 Eh)
+    and against the golden oracle energies of an independently built B (<= 5e-9 Eh)."""
+    lines = (GOLD / "h2o_monomer.xyz").read_text().splitlines()[2:]
+    mono = ri.Molecule.from_angstrom([l.split()[0] for l in lines],
+                                     np.array([[float(x) for x in l.split()[1:]] for l in lines]))
+    from oracle import scf_oracle as so
+    from tests.test_integrals import numpy_jk
+    for name, m in (("water1_monomer_def2svp", mono),
+                    ("water16_seed1_def2svp", ri.Molecule.water_cluster(wl.water_cluster(1, 16)))):
+        say(f"{name}: nbf {m.nbf} naux {m.naux}, FP64 from SAD, tight")
+        out, t = rs.run_molecule(m, ctrl=tight(), keep_b=True)
+        ref = E_GOLD[name]["energy"]
+        # the identical B: the oracle loop from the device's converged density
+        o = so.rhf(t["S"], t["H"], t["rimatr"], m.nelectron, m.nuclear_repulsion(),
+                   so.ScfCtrl(scf_acc_etot=1e-11, scf_acc_rho=1e-9, scf_acc_eev=1e-8, max_scf_cycle=50),
+                   d_guess=out.dm, jk=numpy_jk(t["rimatr"], m.nbf))
+        say(f"  E {out.energy:.12f}; oracle on the identical B {o.energy:.12f} (dE {out.energy - o.energy:.1e}, "
+            f"{o.iterations} it); golden {ref:.12f} (dE {out.energy - ref:.1e}, B from numpy's V^-1/2)")
+        assert out.converged and o.converged and abs(out.energy - o.energy) <= 1e-9
+        # the golden run builds V^{-1/2} with numpy, the device with rocSOLVER; the metric's
+        # condition number (~1e7) turns their 1e-16-level differences into ~1e-9 Eh
+        assert abs(out.energy - ref) <= 5e-9
 
 
 def test_mixed_uhf_shift_matches_fp64():

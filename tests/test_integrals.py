@@ -174,6 +174,11 @@ def test_pentacene_ri_hf_energy_matches_rest():
     assert (m.nbf, m.naux, m.nelectron) == (124, 1902, 146)
     S, T, V = m.int1e()
     B, kept = so.rimatr_from_integrals(m.int3c2e_packed(), m.int2c2e(), 1e-8)
-    res = so.rhf(S, T + V, B, m.nelectron, m.nuclear_repulsion(), jk=numpy_jk(B, m.nbf))
+    # REST's run converged in 19 iterations (examples/PEN/RIHF/rest.out:85), before its energy-rise
+    # check (start_check_oscillation = 20) can act.  From the core guess this loop needs about 53;
+    # with the check on, its mixing steers the SCF to another stationary point (-830.796 Eh, a
+    # HOMO-LUMO gap of 0.04 Eh), so the integrals are pinned here with the check off.
+    ctrl = so.ScfCtrl(start_check_oscillation=1000)
+    res = so.rhf(S, T + V, B, m.nelectron, m.nuclear_repulsion(), ctrl, jk=numpy_jk(B, m.nbf))
     assert res.converged
     assert abs(res.energy - (-831.0241601906)) < 1e-8, res.energy
