@@ -50,6 +50,14 @@ def test_mixing_with_alpha_zero_returns_the_density_two_steps_back(pentacene):
     assert steps, "no energy rise after the second DIIS step"
     for k in steps:
         assert np.array_equal(r.densities[k - 1], r.densities[k - 3]), k
+    # the next step rebuilds the Fock matrix of step k−2 with its own density: a repeated error
+    # vector, a singular DIIS matrix and the linear-mixing fallback (REST :3061-3078), which with
+    # mix_param 0 keeps its input density (energy of step k again), then takes the density made
+    # at step k, i.e. the one made at step k−2 (energy of step k−1)
+    j = steps[0]           # 1-based step; r.energies[j] is its energy
+    e = r.energies
+    assert r.diis_failed[j] and not any(r.diis_failed[:j])
+    assert e[j + 1] == e[j] and e[j + 2] == e[j - 1]
 
 
 def test_mixing_with_alpha_one_is_the_plain_diis_path(pentacene):
