@@ -29,6 +29,7 @@ SOURCES = ["rjk_api.cpp", "rjk_j.hip", "rjk_k.hip", "rjk_util.hip", "rsi_int.cpp
 # across a software-pipelined loop; the default AGPR form makes the allocator
 # copy them VGPR<->AGPR every iteration.
 PER_FILE = {"rjk_k.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"], "rjk_batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+            "rjk_kbf.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
             "rsi_int.cpp": ["-fopenmp"]}
 LIBS = ["-fopenmp", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
 

@@ -21,7 +21,7 @@ constexpr int kNumPhases = 5;        // J-gamma, J-acc, K-half, K-syrk, K-other
 constexpr int kTileMN = 128;         // nbf padding for the K tiles
 constexpr int kTileK = 16;           // k-block of the K tiles
 constexpr int kColPad = 64;          // occupied-column padding of M (all spins)
-constexpr int kBfColGran = 32;       // BF16 K: per-spin column granule (one MFMA k-block)
+constexpr int kBfColGran = 64;       // BF16 K: per-spin column granule (the deepest SYRK k-block)
 constexpr int kBfStoredPlanes = 3;   // BF16 planes kept per B element (RJK_STORE_BF16)
 constexpr int kNumPrec = 4;          // profile rows: one per RJK_PREC_* value
 

@@ -125,22 +125,35 @@ __global__ void k_bf_prep_c_idx(const double* __restrict__ vecs, int nbf, const 
     }
 }
 
-// Tile shapes (measured at nbf 1536, nk 384, 738 auxiliary columns per launch, against
-// 64c × 128μ / 32-deep and 128 × 128 / 32-deep with padded LDS rows, and hipBLASLt's
-// batched BF16 GEMM of the same shape at 883 TFLOP/s): half-transform 931 TFLOP/s,
-// SYRK 981 TFLOP/s of executed MFMA work.
-template <int S> using KhalfCfg = bft::Cfg<S, 64, 128, 2, 2, false, 64>;    // X = 64 c, Y = 128 μ
-template <int S> using KsyrkCfg = bft::Cfg<S, 128, 128, 2, 2, false, 32>;   // X = 128 ν, Y = 128 μ
+// Tile shapes from a sweep at nbf 1536, nk 320, 256 auxiliary columns per launch (executed
+// BF16 MFMA rate, accumulators in VGPRs):
+//   half-transform, 2 planes: 64 c × 256 μ, one wave per 64 × 64, 64-deep: 902 TFLOP/s
+//     (64 × 128 on 2 × 2 waves, 64-deep: 734; 32-deep double-buffered: 737); needs
+//     nbf_pad % 256 == 0, else the 64 × 128 tile;
+//   half-transform, 3 planes: 64 c × 128 μ on 2 × 2 waves, 64-deep: 965 (64 × 256: 848);
+//   SYRK, 2 planes: 128 × 128 on 2 × 2 waves, 64-deep: 968 (32-deep: 944);
+//   SYRK, 3 planes: 128 × 128 on 2 × 4 waves, 32-deep: 1094 (2 × 2 waves: 1074).
+template <int S> struct KbfTiles;
+template <> struct KbfTiles<2> {
+    using Half = bft::Cfg<2, 64, 256, 1, 4, false, 64>;    // X = 64 c, Y = 256 μ
+    using HalfNarrow = bft::Cfg<2, 64, 128, 2, 2, false, 64>;
+    using Syrk = bft::Cfg<2, 128, 128, 2, 2, false, 64>;   // X = 128 ν, Y = 128 μ
+};
+template <> struct KbfTiles<3> {
+    using Half = bft::Cfg<3, 64, 128, 2, 2, false, 64>;
+    using HalfNarrow = Half;
+    using Syrk = bft::Cfg<3, 128, 128, 2, 4, false, 32>;
+};
 
 // ---- stage 1 -------------------------------------------------------------------
 // 1-D grid, XCD-aware: every tile of auxiliary column P runs on XCD P % 8 and the
 // column tiles of one μ tile are consecutive there, so they share B's rows in L2.
-template <int S>
-__global__ __launch_bounds__(KhalfCfg<S>::NT) void k_bf_khalf(const __bf16* __restrict__ bpl, size_t naux, int nbf,
-                                                              int nbf_pad, size_t P0, int Pc, int ntm, int ntc,
-                                                              const __bf16* __restrict__ ct, int nk,
-                                                              __bf16* __restrict__ mpl) {
-    using C = KhalfCfg<S>;
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_bf_khalf(const __bf16* __restrict__ bpl, size_t naux, int nbf,
+                                                    int nbf_pad, size_t P0, int Pc, int ntm, int ntc,
+                                                    const __bf16* __restrict__ ct, int nk,
+                                                    __bf16* __restrict__ mpl) {
+    constexpr int S = C::S;
     __shared__ __bf16 lds[C::LDS_ELEMS];
     const int w = blockIdx.x, xcd = w % kXcdB, kq = w / kXcdB, per = ntm * ntc;
     const int Pl = (kq / per) * kXcdB + xcd;
@@ -177,14 +190,13 @@ __global__ __launch_bounds__(KhalfCfg<S>::NT) void k_bf_khalf(const __bf16* __re
 // ---- stage 2 -------------------------------------------------------------------
 // 1-D grid of nsplit × nspin × ntile2 work-groups (nsplit a multiple of 8; split s
 // on XCD s % 8).  Upper tile pairs (μ tile <= ν tile) of 128 × 128.
-template <int S>
-__global__ __launch_bounds__(KsyrkCfg<S>::NT) void k_bf_ksyrk(const __bf16* __restrict__ mpl, int Pc, int nbf_pad,
-                                                              int nk, int ntile2, int nspin, int off0, int cnt0,
-                                                              int off1, int cnt1,
-                                                              const unsigned short* __restrict__ smask,
-                                                              int p_per_split, int beta, double* __restrict__ kpart,
-                                                              size_t slot) {
-    using C = KsyrkCfg<S>;
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_bf_ksyrk(const __bf16* __restrict__ mpl, int Pc, int nbf_pad,
+                                                    int nk, int ntile2, int nspin, int off0, int cnt0,
+                                                    int off1, int cnt1,
+                                                    const unsigned short* __restrict__ smask,
+                                                    int p_per_split, int beta, double* __restrict__ kpart,
+                                                    size_t slot) {
     __shared__ __bf16 lds[C::LDS_ELEMS];
     const int w = blockIdx.x, xcd = w % kXcdB, kq = w / kXcdB;
     const size_t t = (size_t)(kq % ntile2);
@@ -280,37 +292,48 @@ hipError_t launch_bf_prep_c_idx(hipStream_t st, int prec, const double* vecs, si
     return hipGetLastError();
 }
 
+namespace {
+
+template <class C>
+hipError_t bf_khalf_t(hipStream_t st, const void* bpl, size_t naux, size_t nbf, size_t P0, size_t Pc, const KPlan& kp,
+                      const void* ct, void* mpl) {
+    const int ntm = (int)(kp.nbf_pad / C::WGY), ntc = (int)(kp.nk / C::WGX);
+    const size_t nwg = ((Pc + kXcdB - 1) / kXcdB * kXcdB) * (size_t)ntm * ntc;
+    hipLaunchKernelGGL(k_bf_khalf<C>, dim3((unsigned)nwg), dim3(C::NT), 0, st, (const __bf16*)bpl, naux, (int)nbf,
+                       (int)kp.nbf_pad, P0, (int)Pc, ntm, ntc, (const __bf16*)ct, (int)kp.nk, (__bf16*)mpl);
+    return hipGetLastError();
+}
+
+template <class C>
+hipError_t bf_ksyrk_t(hipStream_t st, const void* mpl, size_t Pc, const KPlan& kp, int nspin, const void* smask,
+                      int beta, double* kpart) {
+    const size_t nt = kp.nbf_pad / C::WGX, nt2 = nt * (nt + 1) / 2;
+    const int pps = (int)((Pc + kp.nsplit - 1) / kp.nsplit);
+    const size_t nwg = kp.nsplit * (size_t)nspin * nt2;
+    hipLaunchKernelGGL(k_bf_ksyrk<C>, dim3((unsigned)nwg), dim3(C::NT), 0, st, (const __bf16*)mpl, (int)Pc,
+                       (int)kp.nbf_pad, (int)kp.nk, (int)nt2, nspin, (int)kp.col_off[0], (int)kp.col_cnt[0],
+                       (int)kp.col_off[1], (int)kp.col_cnt[1], (const unsigned short*)smask, pps, beta, kpart,
+                       kp.nbf_pad * kp.nbf_pad);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Tile preconditions: nbf_pad a multiple of 128 (both SYRK tiles, the narrow half-transform
+// tile) or of 256 (the wide one, chosen only then); nk a multiple of 64 (the c tiles); every
+// spin's column count a multiple of kBfColGran = 64 (the deepest k-block).
 hipError_t launch_bf_khalf(hipStream_t st, int prec, const void* bpl, size_t naux, size_t nbf, size_t P0, size_t Pc,
                            const KPlan& kp, const void* ct, void* mpl) {
-    const int ntm = (int)(kp.nbf_pad / 128), ntc = (int)(kp.nk / 64);
-    const size_t nwg = ((Pc + kXcdB - 1) / kXcdB * kXcdB) * (size_t)ntm * ntc;
-    if (bf_planes(prec) == 3)
-        hipLaunchKernelGGL(k_bf_khalf<3>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)bpl, naux,
-                           (int)nbf, (int)kp.nbf_pad, P0, (int)Pc, ntm, ntc, (const __bf16*)ct, (int)kp.nk,
-                           (__bf16*)mpl);
-    else
-        hipLaunchKernelGGL(k_bf_khalf<2>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)bpl, naux,
-                           (int)nbf, (int)kp.nbf_pad, P0, (int)Pc, ntm, ntc, (const __bf16*)ct, (int)kp.nk,
-                           (__bf16*)mpl);
-    return hipGetLastError();
+    if (bf_planes(prec) == 3) return bf_khalf_t<KbfTiles<3>::Half>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
+    if (kp.nbf_pad % KbfTiles<2>::Half::WGY == 0)
+        return bf_khalf_t<KbfTiles<2>::Half>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
+    return bf_khalf_t<KbfTiles<2>::HalfNarrow>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
 }
 
 hipError_t launch_bf_ksyrk(hipStream_t st, int prec, const void* mpl, size_t Pc, const KPlan& kp, int nspin,
                            const void* smask, int beta, double* kpart) {
-    const size_t nt = kp.nbf_pad / 128, nt2 = nt * (nt + 1) / 2;
-    const int pps = (int)((Pc + kp.nsplit - 1) / kp.nsplit);
-    const size_t nwg = kp.nsplit * (size_t)nspin * nt2;
-    if (bf_planes(prec) == 3)
-        hipLaunchKernelGGL(k_bf_ksyrk<3>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)mpl, (int)Pc,
-                           (int)kp.nbf_pad, (int)kp.nk, (int)nt2, nspin, (int)kp.col_off[0], (int)kp.col_cnt[0],
-                           (int)kp.col_off[1], (int)kp.col_cnt[1], (const unsigned short*)smask, pps, beta, kpart,
-                           kp.nbf_pad * kp.nbf_pad);
-    else
-        hipLaunchKernelGGL(k_bf_ksyrk<2>, dim3((unsigned)nwg), dim3(256), 0, st, (const __bf16*)mpl, (int)Pc,
-                           (int)kp.nbf_pad, (int)kp.nk, (int)nt2, nspin, (int)kp.col_off[0], (int)kp.col_cnt[0],
-                           (int)kp.col_off[1], (int)kp.col_cnt[1], (const unsigned short*)smask, pps, beta, kpart,
-                           kp.nbf_pad * kp.nbf_pad);
-    return hipGetLastError();
+    if (bf_planes(prec) == 3) return bf_ksyrk_t<KbfTiles<3>::Syrk>(st, mpl, Pc, kp, nspin, smask, beta, kpart);
+    return bf_ksyrk_t<KbfTiles<2>::Syrk>(st, mpl, Pc, kp, nspin, smask, beta, kpart);
 }
 
 }  // namespace rjk

@@ -225,6 +225,14 @@ __global__ void k_mix(const double* __restrict__ A, const double* __restrict__ B
     if (t < n2) out[t] = A[t] + alpha * (B[t] - A[t]);
 }
 
+// out = prev·(1 − α) + new·α, REST's form of the energy-rise mixing (src/scf_io/mod.rs:3045-3047):
+// α = 1 gives new and α = 0 gives prev exactly
+__global__ void k_blend(const double* __restrict__ prev, const double* __restrict__ cur, double alpha, size_t n2,
+                        double* __restrict__ out) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n2) out[t] = prev[t] * (1.0 - alpha) + cur[t] * alpha;
+}
+
 // out[:, j] = U[:, j] · w[j]   (column-major n × n)
 __global__ void k_scale_columns(const double* __restrict__ U, const double* __restrict__ w, size_t n,
                                 double* __restrict__ out) {
@@ -982,7 +990,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                 if (osc) {
                     for (int s = 0; s < nspin; ++s) {
                         if (hl[0] >= 0)
-                            hipLaunchKernelGGL(k_mix, g2, b256, 0, st, hprev[hl[0]][s], Fx[s], ctrl.mix_param, n2,
+                            hipLaunchKernelGGL(k_blend, g2, b256, 0, st, hprev[hl[0]][s], Fx[s], ctrl.mix_param, n2,
                                                Fx[s]);
                         else
                             SCF_HIP(hipMemcpyAsync(Fx[s], F[s], n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
