@@ -146,6 +146,19 @@ int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, double enuc,
                     const double* d0, int prec, double* energy, double* d1, double* mo_coeff, double* mo_energy,
                     double* energy_d1);
 
+/* rscf_guess_step with the guess density given also in factored form,
+ * D0_s = Σ_k w0_s[k] c0_s[:,k] c0_s[:,k]ᵀ over the first ncol[s] columns of c0_s
+ * (c0: nspin × nbf × nbf column-major; w0: nspin × nbf): the first K runs by the
+ * occupied-orbital route over those columns instead of factorising D0 on the
+ * device.  A SAD density is block-diagonal by atom, so its factors come from the
+ * atomic blocks (no REST counterpart: REST's guess step takes the density,
+ * src/initial_guess/mod.rs:54-64).  d0 must equal the factored sum; it serves J
+ * and E(D0). */
+int rscf_guess_step_factored(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin,
+                             const size_t* nelec, const double* d0, const double* c0, const double* w0,
+                             const size_t* ncol, int prec, double* energy, double* d1, double* mo_coeff,
+                             double* mo_energy, double* energy_d1);
+
 /* rscf_rhf started from guess orbitals instead of a density: c_guess is
  * nbf×nbf column-major, its first nelec/2 columns occupied.  The first Fock
  * matrix uses the occupied-orbital K, as REST's scf() does after its guess

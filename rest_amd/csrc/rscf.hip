@@ -375,7 +375,8 @@ namespace {
 int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h, double enuc, int nspin,
              const size_t nel[2], const double* d_guess, const rscf_ctrl* ctrl_in, bool guess_only,
              rscf_result* result, double* mo_coeff, double* mo_energy, double* dm_out, double* iter_log,
-             bool energy_only = false, const double* c_guess = nullptr, double* energy_d1 = nullptr) {
+             bool energy_only = false, const double* c_guess = nullptr, double* energy_d1 = nullptr,
+             const double* c0_guess = nullptr, const double* w0_guess = nullptr, const size_t* ncol0 = nullptr) {
     if (!S_h || !H_h) return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": S and H are required");
     rscf_ctrl ctrl;
     rscf_default_ctrl(&ctrl);
@@ -436,7 +437,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     double* info_d = mem.get(ctx, 1, &rc);
     // per spin: Fock, extrapolated Fock, eigenvectors, densities (current, previous,
     // mixed), eigenvalues, occupations, exchange
-    double *F[2], *Fx[2], *A[2], *D[2], *Dp[2], *Dm[2], *W[2], *occd[2], *vk[2], *Fl[2];
+    double *F[2], *Fx[2], *A[2], *D[2], *Dp[2], *Dm[2], *W[2], *occd[2], *vk[2], *Fl[2], *gw[2] = {nullptr, nullptr};
     for (int s = 0; s < nspin; ++s) {
         F[s] = mem.get(ctx, n2, &rc);
         Fl[s] = mem.get(ctx, n2, &rc);      // low-precision Fock at the switch
@@ -448,6 +449,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         W[s] = mem.get(ctx, n, &rc);
         occd[s] = mem.get(ctx, n, &rc);
         vk[s] = mem.get(ctx, npair, &rc);
+        if (c0_guess) gw[s] = mem.get(ctx, n, &rc);   // weights of the factored guess density
     }
     const int nd = ctrl.num_max_diis;
     const size_t ev_len = (size_t)nspin * n2;   // error vector: spins back to back (REST :3133)
@@ -652,6 +654,10 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     // (or after an exact diagonalisation dropped the reference) it is a full FP64 build and
     // becomes the new reference.
     bool inc_armed = false, have_ref = false;
+    // a factored guess density D0_s = Σ_k w_k c_k c_kᵀ: its first K by the occupied route over
+    // those columns (A[s], gw[s], gcols[s]) instead of the density-only route
+    bool guess_cols = false;
+    size_t gcols[2] = {0, 0};
     int n_inc = 0, n_refbuild = 0;
     double dref_max = 0.0, last_dref = 0.0, last_inc_err = 0.0;
     auto fock = [&](double* const* Dsrc, bool occ_route, int prec, double* E) -> int {
@@ -702,6 +708,15 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
                 if (nocc[s] == 0) {
                     hipLaunchKernelGGL(k_fill, dim3((unsigned)((npair + 255) / 256)), b256, 0, st, vk[s], npair, 0.0);
                     SCF_HIP(hipGetLastError());
+                } else if (guess_cols) {
+                    if (gcols[s] == 0) {
+                        hipLaunchKernelGGL(k_fill, dim3((unsigned)((npair + 255) / 256)), b256, 0, st, vk[s], npair,
+                                           0.0);
+                        SCF_HIP(hipGetLastError());
+                    } else {
+                        const size_t homo[2] = {gcols[s] - 1, 0};
+                        SCF_RC(rjk_vk_upper_occ_dev(ctx, A[s], n, homo, gw[s], 1, sk, prec, vk[s]));
+                    }
                 } else if (occ_route) {
                     const size_t homo[2] = {nocc[s] - 1, 0};
                     SCF_RC(rjk_vk_upper_occ_dev(ctx, A[s], n, homo, occd[s], 1, sk, prec, vk[s]));
@@ -766,6 +781,16 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     if (d_guess) {
         for (int s = 0; s < nspin; ++s)
             SCF_HIP(hipMemcpyAsync(Dp[s], d_guess + (size_t)s * n2, n2 * sizeof(double), hipMemcpyHostToDevice, st));
+        if (c0_guess) {
+            for (int s = 0; s < nspin; ++s) {
+                gcols[s] = ncol0[s];
+                SCF_HIP(hipMemcpyAsync(A[s], c0_guess + (size_t)s * n2, n2 * sizeof(double), hipMemcpyHostToDevice,
+                                       st));
+                SCF_HIP(hipMemcpyAsync(gw[s], w0_guess + (size_t)s * n, n * sizeof(double), hipMemcpyHostToDevice,
+                                       st));
+            }
+            guess_cols = true;
+        }
     } else if (c_guess) {   // guess orbitals: their density, first Fock by the occupied route
         for (int s = 0; s < nspin; ++s) {
             SCF_HIP(hipMemcpyAsync(A[s], c_guess + (size_t)s * n2, n2 * sizeof(double), hipMemcpyHostToDevice, st));
@@ -783,6 +808,7 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
     int prec_low = prec;   // the precision the FP64 switch leaves
     double E = 0.0;
     SCF_RC(fock(Dp, guess_from_orbitals, prec, &E));
+    guess_cols = false;
     const double E_guess = E;
     // the guess's own self-consistency error, max|F D S − S D F| over the spins
     double guess_err = 0.0;
@@ -1178,6 +1204,27 @@ extern "C" int rscf_guess_step(rjk_ctx* ctx, const double* S, const double* H, d
     rscf_result r{};
     const int rc = scf_core(ctx, "rscf_guess_step", S, H, enuc, (int)nspin, nel, d0, &c, true, &r, mo_coeff,
                             mo_energy, d1, nullptr, false, nullptr, energy_d1);
+    if (!rc && energy) *energy = r.energy;
+    return rc;
+}
+
+extern "C" int rscf_guess_step_factored(rjk_ctx* ctx, const double* S, const double* H, double enuc, size_t nspin,
+                                        const size_t* nelec, const double* d0, const double* c0, const double* w0,
+                                        const size_t* ncol, int prec, double* energy, double* d1, double* mo_coeff,
+                                        double* mo_energy, double* energy_d1) {
+    if (!ctx) return RJK_ERR_ARG;
+    if (!nelec || !d0 || !c0 || !w0 || !ncol || (nspin != 1 && nspin != 2))
+        return rjk::fail(ctx, RJK_ERR_ARG,
+                         "rscf_guess_step_factored: nspin must be 1 or 2, nelec, d0, c0, w0 and ncol non-NULL");
+    for (size_t s = 0; s < nspin; ++s)
+        if (ncol[s] > ctx->nbf) return rjk::fail(ctx, RJK_ERR_ARG, "rscf_guess_step_factored: ncol exceeds nbf");
+    const size_t nel[2] = {nelec[0], nspin == 2 ? nelec[1] : 0};
+    rscf_ctrl c;
+    rscf_default_ctrl(&c);
+    c.jk_prec = prec;
+    rscf_result r{};
+    const int rc = scf_core(ctx, "rscf_guess_step_factored", S, H, enuc, (int)nspin, nel, d0, &c, true, &r, mo_coeff,
+                            mo_energy, d1, nullptr, false, nullptr, energy_d1, c0, w0, ncol);
     if (!rc && energy) *energy = r.energy;
     return rc;
 }

@@ -100,6 +100,29 @@ def sad_density(mol: "ri.Molecule") -> tuple:
     return da, db
 
 
+def sad_factors(mol: "ri.Molecule", cut: float = 1e-13) -> tuple:
+    """(C0, w0) with D_α + D_β = C0 diag(w0) C0ᵀ, from the eigenpairs of every atomic block
+    (|w| > cut·max|w| of that block); the columns are zero outside their atom's functions."""
+    n = mol.nbf
+    cols, ws = [], []
+    off = 0
+    cache = {}
+    for el in mol.symbols:
+        if el not in cache:
+            a = atomic_density(el, mol.basis, mol.auxbasis)
+            lam, U = np.linalg.eigh(a.d_alpha + a.d_beta)
+            keep = np.abs(lam) > cut * np.abs(lam).max()
+            cache[el] = (lam[keep], U[:, keep])
+        lam, U = cache[el]
+        k = U.shape[0]
+        block = np.zeros((n, U.shape[1]))
+        block[off:off + k] = U
+        cols.append(block)
+        ws.append(lam)
+        off += k
+    return np.concatenate(cols, axis=1), np.concatenate(ws)
+
+
 @dataclass
 class Guess:
     d0: np.ndarray          # assembled guess density (RHF: D_α + D_β)
@@ -113,5 +136,5 @@ def sad_guess(mol: "ri.Molecule", r: "jk.DeviceRI", S: np.ndarray, H: np.ndarray
     """REST's SAD initial guess for a closed-shell molecule whose B is resident in r."""
     da, db = sad_density(mol)
     d0 = da + db
-    e0, d1, c1, _ = rs.guess_step(r, S, H, mol.nuclear_repulsion(), (mol.nelectron,), d0)
+    e0, d1, c1, _ = rs.guess_step(r, S, H, mol.nuclear_repulsion(), (mol.nelectron,), d0, factors=sad_factors(mol))
     return Guess(d0, e0, d1, c1, "sad")

@@ -50,7 +50,8 @@ EXPORTS = [
     "rsi_basis_create", "rsi_basis_destroy", "rsi_basis_nbf", "rsi_last_error", "rsi_int1e", "rsi_int2c2e",
     "rsi_int3c2e_packed", "rsi_int3c2e_packed_dev", "rsi_build_rimatr", "rsi_cart2sph",
     # include/rest_scf.h
-    "rscf_default_ctrl", "rscf_rhf", "rscf_rhf_orbitals", "rscf_uhf", "rscf_guess_step", "rscf_energy", "rscf_mcweeny",
+    "rscf_default_ctrl", "rscf_rhf", "rscf_rhf_orbitals", "rscf_uhf", "rscf_guess_step", "rscf_guess_step_factored",
+    "rscf_energy", "rscf_mcweeny",
     # include/rest_batch.h
     "rjk_batch_create", "rjk_batch_destroy", "rjk_batch_last_error", "rjk_batch_size", "rjk_batch_bytes_b",
     "rjk_batch_set_stream", "rjk_batch_set_rimatr", "rjk_batch_fill_synthetic", "rjk_batch_set_inputs",

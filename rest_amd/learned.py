@@ -414,11 +414,14 @@ def encode(mol: "ri.Molecule", D: np.ndarray, D_sad: np.ndarray):
 
 
 def decode(mol: "ri.Molecule", pairs: np.ndarray, frames_: np.ndarray, Y: np.ndarray, D_sad: np.ndarray):
-    """D = D_sad + Σ predicted pair blocks rotated back, symmetric."""
+    """D = D_sad + Σ predicted pair blocks rotated back, symmetric.  The pairs are unique with
+    A <= B (pair_list), so every matrix element belongs to one block or its mirror: the blocks are
+    written, not accumulated."""
     dD = np.zeros_like(D_sad)
     Y = np.asarray(Y, dtype=np.float64).reshape(len(pairs), MAXB, MAXB) / TARGET_SCALE
     for rows, oa, ob, la, lb in _groups(mol, pairs):
-        Ua, Ub = atom_rotations(la, frames_[rows]), atom_rotations(lb, frames_[rows])
+        Ua = atom_rotations(la, frames_[rows])
+        Ub = Ua if list(lb) == list(la) else atom_rotations(lb, frames_[rows])
         na, nb = Ua.shape[1], Ub.shape[1]
         blk = np.transpose(Ua, (0, 2, 1)) @ Y[rows, :na, :nb] @ Ub
         same = pairs[rows, 0] == pairs[rows, 1]
@@ -426,9 +429,9 @@ def decode(mol: "ri.Molecule", pairs: np.ndarray, frames_: np.ndarray, Y: np.nda
             blk[same] = 0.5 * (blk[same] + np.transpose(blk[same], (0, 2, 1)))
         ia = oa[:, None] + np.arange(na)[None, :]
         ib = ob[:, None] + np.arange(nb)[None, :]
-        np.add.at(dD, (ia[:, :, None], ib[:, None, :]), blk)
+        dD[ia[:, :, None], ib[:, None, :]] = blk
         nd = ~same
-        np.add.at(dD, (ib[nd][:, :, None], ia[nd][:, None, :]), np.transpose(blk[nd], (0, 2, 1)))
+        dD[ib[nd][:, :, None], ia[nd][:, None, :]] = np.transpose(blk[nd], (0, 2, 1))
     return D_sad + dD
 
 
@@ -683,7 +686,8 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
     enuc = mol.nuclear_repulsion()
     e_s = None
     if guard == "energy":
-        e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True, want_outputs=False)[4]
+        e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True, want_outputs=False,
+                            factors=gs.sad_factors(mol))[4]
     t4 = time.perf_counter()
     ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
     out, e_l, err_l = None, float("inf"), None

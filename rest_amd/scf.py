@@ -57,6 +57,9 @@ def _bind(lib):
     lib.rscf_guess_step.restype = C.c_int
     lib.rscf_guess_step.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, C.c_int, C.POINTER(C.c_double), vp,
                                     vp, vp, C.POINTER(C.c_double)]
+    lib.rscf_guess_step_factored.restype = C.c_int
+    lib.rscf_guess_step_factored.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, vp, vp, vp, C.c_int,
+                                             C.POINTER(C.c_double), vp, vp, vp, C.POINTER(C.c_double)]
     lib.rscf_energy.restype = C.c_int
     lib.rscf_energy.argtypes = [vp, vp, vp, C.c_double, C.c_size_t, vp, vp, C.c_int, C.POINTER(C.c_double)]
     lib.rscf_mcweeny.restype = C.c_int
@@ -176,10 +179,11 @@ def uhf(ri: "jk.DeviceRI", S: np.ndarray, H: np.ndarray, enuc: float, nalpha: in
 
 
 def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PREC_FP64, want_e1: bool = False,
-               want_outputs: bool = True):
+               want_outputs: bool = True, factors=None):
     """REST's density-guess step (rscf_guess_step): returns (E(D0), D1, C, ε), plus E(D1) when
     want_e1; nel = (nelec,) or (nα, nβ).  With want_outputs=False D1, C and ε stay on the device
-    and come back as None (the energies only)."""
+    and come back as None (the energies only).  factors: per spin (C0, w0) with D0 = C0 diag(w0) C0ᵀ
+    (rscf_guess_step_factored: the first K by the occupied route over C0's columns)."""
     lib = _bind(jk.load_library())
     n = ri.nbf
     ns = len(nel)
@@ -193,8 +197,22 @@ def guess_step(ri: "jk.DeviceRI", S, H, enuc: float, nel, d0, prec: int = jk.PRE
     e = C.c_double(0.0)
     e1 = C.c_double(0.0)
     Sf, Hf = _sym(S), _sym(H)      # kept alive across the call
-    rc = lib.rscf_guess_step(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, d0f.ctypes.data, int(prec),
-                             C.byref(e), ptr(d1), ptr(C_), ptr(eps), C.byref(e1) if want_e1 else None)
+    if factors is None:
+        rc = lib.rscf_guess_step(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, d0f.ctypes.data,
+                                 int(prec), C.byref(e), ptr(d1), ptr(C_), ptr(eps), C.byref(e1) if want_e1 else None)
+    else:
+        fs = [factors] if ns == 1 and len(factors) == 2 and np.ndim(factors[0]) == 2 else list(factors)
+        c0 = np.zeros((ns, n, n))
+        w0 = np.zeros((ns, n))
+        nc = (C.c_size_t * 2)(0, 0)
+        for s_, (cs, ws) in enumerate(fs):
+            k = cs.shape[1]
+            c0[s_, :k, :] = np.asarray(cs, dtype=np.float64).T       # column-major: column j at row j
+            w0[s_, :k] = ws
+            nc[s_] = k
+        rc = lib.rscf_guess_step_factored(ri.h, Sf.ctypes.data, Hf.ctypes.data, float(enuc), ns, nl, d0f.ctypes.data,
+                                          c0.ctypes.data, w0.ctypes.data, C.cast(nc, C.c_void_p), int(prec),
+                                          C.byref(e), ptr(d1), ptr(C_), ptr(eps), C.byref(e1) if want_e1 else None)
     if rc != 0:
         raise jk.RJKError(rc, lib.rjk_last_error(ri.h).decode())
     if not want_outputs:

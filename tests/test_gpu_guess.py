@@ -138,6 +138,34 @@ def test_sad_pentacene_matches_rest_trace():
     np.testing.assert_allclose(e_log, trace["iteration_energies"], rtol=0, atol=1e-7)
 
 
+def test_factored_sad_guess_step_equals_density_route():
+    """rscf_guess_step_factored (the SAD density's first K over its atomic eigenvectors) against
+    rscf_guess_step (the device factorisation of the whole density), FP64 and BF16X2, (H2O)4."""
+    say("(H2O)4 SAD guess step: atomic factors vs the density-only route")
+    m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))
+    S, T, V = m.int1e()
+    H = T + V
+    da, db = gs.sad_density(m)
+    C0, w0 = gs.sad_factors(m)
+    np.testing.assert_allclose((C0 * w0) @ C0.T, da + db, atol=1e-13)
+    assert C0.shape[1] < m.nbf
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        ref = None
+        for prec in (jk.PREC_FP64, jk.PREC_BF16X2):
+            e0, d1, _, _, e1 = rs.guess_step(r, S, H, m.nuclear_repulsion(), (m.nelectron,), da + db, prec,
+                                             want_e1=True)
+            f0, f1, _, _, g1 = rs.guess_step(r, S, H, m.nuclear_repulsion(), (m.nelectron,), da + db, prec,
+                                             want_e1=True, factors=(C0, w0))
+            say(f"  prec {prec}: E(D0) {e0:.10f} vs {f0:.10f}, E(D1) {e1:.10f} vs {g1:.10f}, "
+                f"max|dD1| {np.abs(d1 - f1).max():.1e}")
+            if ref is None:   # FP64: the same numbers through either route
+                ref = (e0, e1)
+                assert abs(e0 - f0) <= 1e-10 and abs(e1 - g1) <= 1e-10 and np.abs(d1 - f1).max() <= 1e-10
+            else:             # BF16X2: each route within the two-plane error of FP64 (different roundings)
+                assert all(abs(x - ref[0]) <= 5e-4 for x in (e0, f0)) and all(abs(x - ref[1]) <= 5e-4 for x in (e1, g1))
+
+
 def test_sad_water_cluster_beats_core_guess():
     say("(H2O)4 SAD vs core-Hamiltonian guess")
     m = ri.Molecule.water_cluster(wl.water_cluster(4, 4))

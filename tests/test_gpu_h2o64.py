@@ -45,7 +45,8 @@ def read_xyz(path):
 def plain_scf(mol, r, S, H):
     """FP64 from SAD: REST's guess step, then the SCF from those orbitals."""
     da, db = gs.sad_density(mol)
-    _, _, c1, _ = rs.guess_step(r, S, H, mol.nuclear_repulsion(), (mol.nelectron,), da + db, jk.PREC_FP64)
+    _, _, c1, _ = rs.guess_step(r, S, H, mol.nuclear_repulsion(), (mol.nelectron,), da + db, jk.PREC_FP64,
+                                factors=gs.sad_factors(mol))
     return rs.rhf(r, S, H, mol.nuclear_repulsion(), mol.nelectron, None, rs.default_ctrl(), c_guess=c1)
 
 
