@@ -45,7 +45,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 MFMA_PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}   # MI355X dense matrix peaks per instruction type
 # MFMA products per algorithmic product on the split-BF16 planes (a + b < S)
-BF16_PRODUCTS = {"bf16x2": 3, "bf16x3": 6}
+BF16_PRODUCTS = {"bf16x1": 1, "bf16x2": 3, "bf16x3": 6}
 
 PHASE_KERNEL = {"j_gamma": "k_jgamma", "j_acc": "k_jacc", "k_half": "k_khalf", "k_syrk": "k_ksyrk"}
 
@@ -59,9 +59,10 @@ def parse():
     ap.add_argument("--clusters", type=int, default=512, help="batched mode: number of water clusters")
     ap.add_argument("--workload", default="h2o64", choices=["h2o64", "h2o16"])
     ap.add_argument("--model", default=str(ROOT / "rest_amd" / "models" / "learned_guess_h2o.npz"))
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x2"],
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x2", "bf16x1"],
                     help="jk mode: fp64 plain path; fp32: FP32 copy of B, FP32 MFMA K, FP64 accumulation of J; "
-                         "bf16x3 / bf16x2: K on three / two BF16 planes of B (6 / 3 MFMA products), J on the FP32 copy")
+                         "bf16x3 / bf16x2 / bf16x1: K on three / two / one BF16 planes of B (6 / 3 / 1 MFMA products), "
+                         "J on the FP32 copy")
     ap.add_argument("--mixed-prec", default="bf16x3", choices=["fp32", "bf16x3"],
                     help="scf mode: J/K precision of the mixed SCF before its switch to FP64")
     ap.add_argument("--pseudo-diag-err", type=float, default=1e-2,
@@ -70,6 +71,8 @@ def parse():
                     help="scf mode: precision of the J/K increments on the FP64 reference build after the switch")
     ap.add_argument("--guard", default="error", choices=["error", "energy"],
                     help="scf mode: learned-guess acceptance (guess-build commutator, or energy against SAD's E(D1))")
+    ap.add_argument("--guard-prec", default="bf16x1", choices=["fp64", "fp32", "bf16x1", "bf16x2", "bf16x3"],
+                    help="scf mode: J/K precision of SAD's E(D1) for the energy rule")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -198,17 +201,21 @@ def main_scf(args):
         out = rs.rhf(r, S, H, enuc, mol.nelectron, None, rs.default_ctrl(jk_prec=jk.PREC_FP64), c_guess=c1)
         return out, {"guess": "sad", "e_guess": e0}
 
+    guard_prec = jk.PREC_NAMES[args.guard_prec]
+
     def mixed(guard=args.guard, increment_prec=args.increment_prec):
         ctrl = rs.mixed_ctrl(jk_prec=mixed_prec, pseudo_diag_err=args.pseudo_diag_err,
                              jk_prec_increment=-1 if increment_prec == "none" else jk.PREC_NAMES[increment_prec])
         if not args.mixed_prec.startswith("bf16"):
             ctrl.jk_prec_start = -1                    # FP32 throughout the low-precision phase
-        g = lg.learned_scf(net, mol, r, S, H, ctrl, guard=guard)
+        g = lg.learned_scf(net, mol, r, S, H, ctrl, guard=guard, guard_prec=guard_prec)
         return g.outcome, {"guess": g.kind, "guard": guard, "e_learned": g.e_learned, "e_sad": g.e_sad,
                            "err_learned": g.err_learned, "seconds": g.seconds,
                            "incremental_iterations": g.outcome.incremental_iterations,
                            "precisions": [int(x["prec"]) for x in g.outcome.log],
-                           "diag_ms": [round(float(x["diag_ms"]), 2) for x in g.outcome.log]}
+                           "diag_ms": [round(float(x["diag_ms"]), 2) for x in g.outcome.log],
+                           "log": [{k: float(x[k]) for k in ("dE", "diis_max", "dm_err", "eev_err", "jk_ms", "iter_ms",
+                                                             "dref")} for x in g.outcome.log]}
 
     # the target's rules read literally: the learned guess kept only if its first energy lies below
     # SAD's E(D1) (computed first), FP64 J/K after the DIIS switch (one full FP64 reference build,
@@ -246,7 +253,7 @@ def main_scf(args):
     t_plain, t_mixed, t_literal = float(np.median(tp)), float(np.median(tm)), float(np.median(tl))
     # the energy guard's own error: SAD's E(D1) at the guard's precision against FP64
     da, db = gs.sad_density(mol)
-    e1_guard = rs.guess_step(r, S, H, enuc, (mol.nelectron,), da + db, jk.PREC_BF16X2, want_e1=True,
+    e1_guard = rs.guess_step(r, S, H, enuc, (mol.nelectron,), da + db, guard_prec, want_e1=True,
                              want_outputs=False, factors=gs.sad_factors(mol))[4]
     e1_fp64 = rs.guess_step(r, S, H, enuc, (mol.nelectron,), da + db, jk.PREC_FP64, want_e1=True,
                             want_outputs=False, factors=gs.sad_factors(mol))[4]
@@ -331,7 +338,8 @@ def main_scf(args):
                                           "converged": ol.converged, "jk_s": ol.jk_seconds, "diag_s": ol.diag_seconds,
                                           **il,
                                           "guard_margin_eh": (il["e_sad"] - il["e_learned"]) if il["e_sad"] else None,
-                                          "guard_e1_bf16x2": e1_guard, "guard_e1_fp64": e1_fp64,
+                                          "guard_margin_vs_fp64_e1_eh": e1_fp64 - il["e_learned"],
+                                          "guard_prec": args.guard_prec, "guard_e1": e1_guard, "guard_e1_fp64": e1_fp64,
                                           "guard_e1_error_eh": e1_guard - e1_fp64},
             "dE_mixed_minus_plain": om.energy - op.energy,
             "dE_energy_rule_minus_plain": ol.energy - op.energy,
@@ -533,7 +541,7 @@ def main_jk(args):
         ach, peak, unit = kmult * per_launch / sec / 1e12, kpeak, "TFLOP/s"
     kname = PHASE_KERNEL[dom]
     if bf and dom.startswith("k_"):
-        kname = kname.replace("k_", "k_bf_", 1) + f"<{3 if args.prec == 'bf16x3' else 2}>"
+        kname = kname.replace("k_", "k_bf_", 1) + f"<{args.prec[-1]}>"
     roofline = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
                 "traffic": load_traffic(kname), "kernel": kname, "algorithmic_per_launch": per_launch}
     # secondary figures: J against HBM, K (both stages) against FP64 MFMA
@@ -554,7 +562,7 @@ def main_jk(args):
     out = {
         "metric": f"RI-V J/K builds/s (one J + one K), {shape.name} shape, "
                   + {"fp64": "plain FP64", "fp32": "FP32 B + FP32 MFMA", "bf16x3": "K on 3 BF16 planes, FP32 J",
-                     "bf16x2": "K on 2 BF16 planes, FP32 J"}[args.prec],
+                     "bf16x2": "K on 2 BF16 planes, FP32 J", "bf16x1": "K on 1 BF16 plane, FP32 J"}[args.prec],
         "value": args.steps / (ms_total / 1e3),
         "unit": "J/K builds/s",
         "n_gpus": 1,

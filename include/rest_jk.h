@@ -54,7 +54,10 @@ enum {
     RJK_PREC_FP64 = 0,   /* FP64 loads, FP64 MFMA / FMA, FP64 accumulation          */
     RJK_PREC_FP32 = 1,   /* FP32 copy of B; J: FP64 accumulation                    */
     RJK_PREC_BF16X2 = 2, /* K: two BF16 planes per operand, 3 products (~2^-16)     */
-    RJK_PREC_BF16X3 = 3  /* K: three BF16 planes per operand, 6 products (~2^-24)   */
+    RJK_PREC_BF16X3 = 3, /* K: three BF16 planes per operand, 6 products (~2^-24)   */
+    RJK_PREC_BF16X1 = 4, /* K: one BF16 plane per operand, 1 product (~2^-8): cheap
+                          * estimates (the energy rule of a guess), not SCF steps   */
+    RJK_NUM_PREC = 5     /* rows of rjk_profile_read_detail                         */
     /* The BF16 precisions run K on BF16 MFMA with FP32 accumulation and FP64
      * split sums; J then runs on the FP32 copy of B when it is stored, else on
      * the FP64 copy. */
@@ -189,8 +192,8 @@ int rjk_vk_upper_dm_dev(rjk_ctx* ctx, const double* dm_dev, size_t spin_channel,
  * launches[5] and resets the counters. */
 int rjk_profile_enable(rjk_ctx* ctx, int on);
 int rjk_profile_read(rjk_ctx* ctx, double* ms, uint64_t* launches);
-/* The same counters split by arithmetic, [4][5] row-major (row = the
- * RJK_PREC_* value: FP64, FP32, BF16X2, BF16X3), plus the algorithmic work of those launches: bytes for the J
+/* The same counters split by arithmetic, [RJK_NUM_PREC][5] row-major (row = the
+ * RJK_PREC_* value: FP64, FP32, BF16X2, BF16X3, BF16X1), plus the algorithmic work of those launches: bytes for the J
  * phases (npair·naux·s_B + 8·nspin·npair per pass), FLOP for the K phases
  * (2·nbf²·nk·P per half-transform, nbf(nbf+1)·nk·P per SYRK), 0 for phase 4.
  * Resets the counters like rjk_profile_read. */

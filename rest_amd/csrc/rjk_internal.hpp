@@ -23,9 +23,11 @@ constexpr int kTileK = 16;           // k-block of the K tiles
 constexpr int kColPad = 64;          // occupied-column padding of M (all spins)
 constexpr int kBfColGran = 64;       // BF16 K: per-spin column granule (the deepest SYRK k-block)
 constexpr int kBfStoredPlanes = 3;   // BF16 planes kept per B element (RJK_STORE_BF16)
-constexpr int kNumPrec = 4;          // profile rows: one per RJK_PREC_* value
+constexpr int kNumPrec = RJK_NUM_PREC; // profile rows: one per RJK_PREC_* value
 
-inline bool is_bf16(int prec) { return prec == RJK_PREC_BF16X2 || prec == RJK_PREC_BF16X3; }
+inline bool is_bf16(int prec) {
+    return prec == RJK_PREC_BF16X1 || prec == RJK_PREC_BF16X2 || prec == RJK_PREC_BF16X3;
+}
 inline int prof_row(int prec) { return prec >= 0 && prec < kNumPrec ? prec : 0; }
 
 inline size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }

@@ -5,6 +5,7 @@
 // and a product uses the plane pairs whose plane indices sum to less than S:
 //   S = 2:  a0·b0 + a0·b1 + a1·b0                        (3 MFMAs, ~2^-16 relative)
 //   S = 3:  a0·b0 + a0·b1 + a1·b0 + a0·b2 + a1·b1 + a2·b0 (6 MFMAs, ~2^-24 relative)
+//   S = 1:  a0·b0                                         (1 MFMA, ~2^-8 relative)
 // accumulated in FP32 by v_mfma_f32_16x16x32_bf16.  Lane l supplies
 // A[l%16][8(l/16)+i] and B[8(l/16)+i][l%16]; register r of lane l receives
 // D[4(l/16)+r][l%16] (probed on gfx950).  Split-K partials join an FP64 sum.
@@ -139,6 +140,11 @@ template <> struct KbfTiles<2> {
     using HalfNarrow = bft::Cfg<2, 64, 128, 2, 2, false, 64>;
     using Syrk = bft::Cfg<2, 128, 128, 2, 2, false, 64>;   // X = 128 ν, Y = 128 μ
 };
+template <> struct KbfTiles<1> {
+    using Half = bft::Cfg<1, 64, 256, 1, 4, false, 64>;
+    using HalfNarrow = bft::Cfg<1, 64, 128, 2, 2, false, 64>;
+    using Syrk = bft::Cfg<1, 128, 128, 2, 2, false, 64>;
+};
 template <> struct KbfTiles<3> {
     using Half = bft::Cfg<3, 64, 128, 2, 2, false, 64>;
     using HalfNarrow = Half;
@@ -245,7 +251,7 @@ unsigned gs_blocks(size_t n) { return (unsigned)std::max<size_t>(1, std::min<siz
 
 }  // namespace
 
-int bf_planes(int prec) { return prec == RJK_PREC_BF16X3 ? 3 : 2; }
+int bf_planes(int prec) { return prec == RJK_PREC_BF16X3 ? 3 : (prec == RJK_PREC_BF16X1 ? 1 : 2); }
 
 hipError_t launch_bf_planes_b(hipStream_t st, const double* b64, size_t ldb, size_t nbf, size_t nbf_pad,
                               size_t naux, void* planes) {
@@ -265,14 +271,16 @@ hipError_t launch_bf_prep_c(hipStream_t st, int prec, const double* eigv, size_t
                             const size_t* nw, const double* occ, int nspin, const KPlan& kp, void* ct, void* smask) {
     const size_t n = kp.nk * kp.nbf_pad;
     const int nw1 = nspin == 2 ? (int)nw[1] : 0;
-    if (bf_planes(prec) == 3)
-        hipLaunchKernelGGL(k_bf_prep_c<3>, dim3(gs_blocks(n)), dim3(256), 0, st, eigv, (int)nbf, (int)nstate,
-                           (int)nw[0], nw1, occ, nspin, (int)kp.col_off[0], (int)kp.col_off[1], (int)kp.nk,
-                           (int)kp.nbf_pad, (__bf16*)ct, (unsigned short*)smask);
-    else
-        hipLaunchKernelGGL(k_bf_prep_c<2>, dim3(gs_blocks(n)), dim3(256), 0, st, eigv, (int)nbf, (int)nstate,
-                           (int)nw[0], nw1, occ, nspin, (int)kp.col_off[0], (int)kp.col_off[1], (int)kp.nk,
-                           (int)kp.nbf_pad, (__bf16*)ct, (unsigned short*)smask);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(gs_blocks(n)), dim3(256), 0, st, eigv, (int)nbf, (int)nstate, (int)nw[0], nw1,
+                           occ, nspin, (int)kp.col_off[0], (int)kp.col_off[1], (int)kp.nk, (int)kp.nbf_pad,
+                           (__bf16*)ct, (unsigned short*)smask);
+    };
+    switch (bf_planes(prec)) {
+        case 3: go(k_bf_prep_c<3>); break;
+        case 1: go(k_bf_prep_c<1>); break;
+        default: go(k_bf_prep_c<2>); break;
+    }
     return hipGetLastError();
 }
 
@@ -281,14 +289,15 @@ hipError_t launch_bf_prep_c_idx(hipStream_t st, int prec, const double* vecs, si
                                 void* smask) {
     if (ncol == 0) return hipSuccess;
     const size_t n = ncol * kp.nbf_pad;
-    if (bf_planes(prec) == 3)
-        hipLaunchKernelGGL(k_bf_prep_c_idx<3>, dim3(gs_blocks(n)), dim3(256), 0, st, vecs, (int)nbf, idx, lam,
-                           (int)ncol, (int)col_off, (int)kp.nk, (int)kp.nbf_pad, (__bf16*)ct,
-                           (unsigned short*)smask);
-    else
-        hipLaunchKernelGGL(k_bf_prep_c_idx<2>, dim3(gs_blocks(n)), dim3(256), 0, st, vecs, (int)nbf, idx, lam,
-                           (int)ncol, (int)col_off, (int)kp.nk, (int)kp.nbf_pad, (__bf16*)ct,
-                           (unsigned short*)smask);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(gs_blocks(n)), dim3(256), 0, st, vecs, (int)nbf, idx, lam, (int)ncol,
+                           (int)col_off, (int)kp.nk, (int)kp.nbf_pad, (__bf16*)ct, (unsigned short*)smask);
+    };
+    switch (bf_planes(prec)) {
+        case 3: go(k_bf_prep_c_idx<3>); break;
+        case 1: go(k_bf_prep_c_idx<1>); break;
+        default: go(k_bf_prep_c_idx<2>); break;
+    }
     return hipGetLastError();
 }
 
@@ -324,16 +333,23 @@ hipError_t bf_ksyrk_t(hipStream_t st, const void* mpl, size_t Pc, const KPlan& k
 // spin's column count a multiple of kBfColGran = 64 (the deepest k-block).
 hipError_t launch_bf_khalf(hipStream_t st, int prec, const void* bpl, size_t naux, size_t nbf, size_t P0, size_t Pc,
                            const KPlan& kp, const void* ct, void* mpl) {
-    if (bf_planes(prec) == 3) return bf_khalf_t<KbfTiles<3>::Half>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
-    if (kp.nbf_pad % KbfTiles<2>::Half::WGY == 0)
-        return bf_khalf_t<KbfTiles<2>::Half>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
-    return bf_khalf_t<KbfTiles<2>::HalfNarrow>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
+    const int S = bf_planes(prec);
+    if (S == 3) return bf_khalf_t<KbfTiles<3>::Half>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
+    const bool wide = kp.nbf_pad % KbfTiles<2>::Half::WGY == 0;
+    if (S == 1)
+        return wide ? bf_khalf_t<KbfTiles<1>::Half>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl)
+                    : bf_khalf_t<KbfTiles<1>::HalfNarrow>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
+    return wide ? bf_khalf_t<KbfTiles<2>::Half>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl)
+                : bf_khalf_t<KbfTiles<2>::HalfNarrow>(st, bpl, naux, nbf, P0, Pc, kp, ct, mpl);
 }
 
 hipError_t launch_bf_ksyrk(hipStream_t st, int prec, const void* mpl, size_t Pc, const KPlan& kp, int nspin,
                            const void* smask, int beta, double* kpart) {
-    if (bf_planes(prec) == 3) return bf_ksyrk_t<KbfTiles<3>::Syrk>(st, mpl, Pc, kp, nspin, smask, beta, kpart);
-    return bf_ksyrk_t<KbfTiles<2>::Syrk>(st, mpl, Pc, kp, nspin, smask, beta, kpart);
+    switch (bf_planes(prec)) {
+        case 3: return bf_ksyrk_t<KbfTiles<3>::Syrk>(st, mpl, Pc, kp, nspin, smask, beta, kpart);
+        case 1: return bf_ksyrk_t<KbfTiles<1>::Syrk>(st, mpl, Pc, kp, nspin, smask, beta, kpart);
+        default: return bf_ksyrk_t<KbfTiles<2>::Syrk>(st, mpl, Pc, kp, nspin, smask, beta, kpart);
+    }
 }
 
 }  // namespace rjk

@@ -402,11 +402,11 @@ int scf_core(rjk_ctx* ctx, const char* who, const double* S_h, const double* H_h
         return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": num_max_diis must be 1..16 and max_scf_cycle >= 1");
     if (!ctx->b64) return rjk::fail(ctx, RJK_ERR_STATE, std::string(who) + ": B is not set");
     if (ctrl.jk_prec != RJK_PREC_FP64 && ctrl.jk_prec != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec))
-        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64, FP32, BF16X2 or BF16X3");
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec must be FP64, FP32 or a BF16 precision");
     if (ctrl.jk_prec_start != -1 && ctrl.jk_prec_start != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec_start))
-        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec_start must be -1, FP32, BF16X2 or BF16X3");
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec_start must be -1, FP32 or a BF16 precision");
     if (ctrl.jk_prec_increment != -1 && ctrl.jk_prec_increment != RJK_PREC_FP32 && !rjk::is_bf16(ctrl.jk_prec_increment))
-        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec_increment must be -1, FP32, BF16X2 or BF16X3");
+        return rjk::fail(ctx, RJK_ERR_ARG, std::string(who) + ": jk_prec_increment must be -1, FP32 or a BF16 precision");
     for (int pr : {ctrl.jk_prec, ctrl.jk_prec_start, ctrl.jk_prec_increment}) {
         if (pr == RJK_PREC_FP32 && !ctx->b32)
             return rjk::fail(ctx, RJK_ERR_STATE,

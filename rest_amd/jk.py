@@ -33,9 +33,11 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "librest_jk.so"
 
 RJK_OK = 0
-PREC_FP64, PREC_FP32, PREC_BF16X2, PREC_BF16X3 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_BF16X2, PREC_BF16X3, PREC_BF16X1 = 0, 1, 2, 3, 4
+NUM_PREC = 5
 STORE_FP64, STORE_FP32, STORE_BF16 = 1, 2, 4
-PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x2": PREC_BF16X2, "bf16x3": PREC_BF16X3}
+PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x2": PREC_BF16X2, "bf16x3": PREC_BF16X3,
+              "bf16x1": PREC_BF16X1}
 ERRORS = {1: "ARG", 2: "HIP", 3: "NOMEM", 4: "STATE", 5: "NONFINITE", 6: "NOSOLVER"}
 
 # every symbol include/rest_jk.h declares (checked by tests/test_abi.py)
@@ -328,13 +330,15 @@ class DeviceRI:
         return {name: (ms[k], int(n[k])) for k, name in enumerate(self.PHASES)}
 
     def profile_read_detail(self) -> dict:
-        """{(prec, phase): (ms, launches, algorithmic work)} with prec "fp64" / "fp32" / "bf16x2" / "bf16x3"."""
-        ms = (C.c_double * 20)()
-        n = (C.c_uint64 * 20)()
-        w = (C.c_double * 20)()
+        """{(prec, phase): (ms, launches, algorithmic work)} with prec "fp64" / "fp32" / "bf16x2" / "bf16x3" /
+        "bf16x1"."""
+        ms = (C.c_double * (5 * NUM_PREC))()
+        n = (C.c_uint64 * (5 * NUM_PREC))()
+        w = (C.c_double * (5 * NUM_PREC))()
         self._check(self.lib.rjk_profile_read_detail(self.h, ms, n, w))
         return {(p, name): (ms[5 * pi + k], int(n[5 * pi + k]), w[5 * pi + k])
-                for pi, p in enumerate(("fp64", "fp32", "bf16x2", "bf16x3")) for k, name in enumerate(self.PHASES)}
+                for pi, p in enumerate(("fp64", "fp32", "bf16x2", "bf16x3", "bf16x1"))
+                for k, name in enumerate(self.PHASES)}
 
     def copy_dev(self, dst_ptr: int, src_ptr: int, nbytes: int):
         self._check(self.lib.rjk_copy_dev(self.h, C.c_void_p(dst_ptr), C.c_void_p(src_ptr), int(nbytes)))

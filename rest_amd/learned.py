@@ -656,7 +656,7 @@ class LearnedScf:
 
 def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
                 mcweeny_iterations: int = 6, guard: str = "error",
-                guess_err_max: float = GUESS_ERR_MAX) -> LearnedScf:
+                guess_err_max: float = GUESS_ERR_MAX, guard_prec: int = None) -> LearnedScf:
     """RHF from the learned guess, with the acceptance rule folded into the SCF.
 
     The SCF starts from the purified prediction; its guess Fock build F(D_pred) decides, before
@@ -666,9 +666,10 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
       guard "error" (default): kept when the guess build's self-consistency error
         max|F D S - S D F| is below guess_err_max; the test costs two GEMMs.
       guard "energy": SAD's E(D1) is computed first (REST's guess step from the SAD density plus
-        the energy of the new orbitals' density, at the SCF's starting precision); kept when
-        E(D_pred) is below it.  Both are energies of idempotent densities with the molecule's
-        electron count, so both bound the converged energy from above and compare fairly.
+        the energy of the new orbitals' density, at guard_prec, by default the SCF's starting
+        precision); kept when E(D_pred) is below it.  Both are energies of idempotent densities with
+        the molecule's electron count, so both bound the converged energy from above and compare
+        fairly.
     A prediction that is not finite or misses the electron count by 0.5 or more is never kept."""
     if guard not in ("error", "energy"):
         raise ValueError(f"learned_scf: guard must be 'error' or 'energy', not {guard!r}")
@@ -686,8 +687,8 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
     enuc = mol.nuclear_repulsion()
     e_s = None
     if guard == "energy":
-        e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0, want_e1=True, want_outputs=False,
-                            factors=gs.sad_factors(mol))[4]
+        e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0 if guard_prec is None else guard_prec,
+                            want_e1=True, want_outputs=False, factors=gs.sad_factors(mol))[4]
     t4 = time.perf_counter()
     ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
     out, e_l, err_l = None, float("inf"), None

@@ -272,7 +272,7 @@ def run_molecule(mol, prec: int = jk.PREC_FP64, d_guess=None, ctrl: Optional[Ctr
     t1 = time.perf_counter()
     ri_ = jk.DeviceRI(mol.nbf, mol.naux)
     store = jk.STORE_FP64 | (jk.STORE_FP32 if prec != jk.PREC_FP64 else 0) \
-        | (jk.STORE_BF16 if prec in (jk.PREC_BF16X2, jk.PREC_BF16X3) else 0)
+        | (jk.STORE_BF16 if prec in (jk.PREC_BF16X1, jk.PREC_BF16X2, jk.PREC_BF16X3) else 0)
     binfo = mol.build_rimatr(ri_, threshold, store)
     t2 = time.perf_counter()
     c_guess = None

@@ -55,6 +55,24 @@ def test_bf16x2_matches_split_oracle_and_x3_is_fp32_level():
     assert e3 < REL3
 
 
+def test_bf16x1_matches_split_oracle():
+    """One BF16 plane per operand, one product: the device equals the one-plane emulation up to FP32
+    accumulation (a tenth of the emulation's distance from FP64), and sits near 2^-8 of FP64."""
+    nbf, naux, nocc = 130, 40, 17
+    say(f"BF16X1 split K nbf={nbf} naux={naux} nocc={nocc}")
+    b = orc.synth_rimatr(21, nbf, naux)
+    c = orc.random_orthonormal(5, nbf, nocc)
+    occ = np.full(nocc, 2.0)
+    ref = orc.vk_upper_occ(b, True, [c], [nocc - 1], [occ], 1, -0.5)[0]
+    emul1 = so.vk_split(b, c, occ, 1, -0.5)
+    with jk.DeviceRI(nbf, naux) as ri:
+        ri.fill_synthetic(21, STORE)
+        k1 = ri.vk_occ([c], [nocc - 1], [occ], 1, -0.5, prec=jk.PREC_BF16X1)[0]
+    e_oracle, e_dev = rel(emul1, ref), rel(k1, emul1)
+    say(f"  x1: oracle vs FP64 {e_oracle:.2e}, device vs oracle {e_dev:.2e}")
+    assert e_dev < 0.1 * e_oracle and e_oracle < 2e-2
+
+
 @pytest.mark.parametrize("nbf,naux,nw,ns", [
     (1, 1, (1, 1), 1), (5, 3, (2, 1), 2), (17, 29, (3, 3), 1), (130, 77, (17, 5), 2),
     (48, 226, (10, 9), 2), (129, 40, (33, 31), 1), (200, 50, (40, 1), 2),
@@ -158,12 +176,12 @@ def test_profile_rows_per_precision():
         ri.fill_synthetic(21, STORE)
         ri.profile_enable(True)
         ri.profile_read_detail()
-        for p in (jk.PREC_FP64, jk.PREC_BF16X3, jk.PREC_BF16X2):
+        for p in (jk.PREC_FP64, jk.PREC_BF16X3, jk.PREC_BF16X2, jk.PREC_BF16X1):
             ri.vk_occ([c], [nocc - 1], [occ], 1, 1.0, prec=p)
         ri.vj([c @ c.T], 1, 1.0, prec=jk.PREC_BF16X3)
         d = ri.profile_read_detail()
     nk = 64   # 17 columns, padded to the column granule and then to 64
-    for name in ("fp64", "bf16x3", "bf16x2"):
+    for name in ("fp64", "bf16x3", "bf16x2", "bf16x1"):
         ms, n, work = d[(name, "k_half")]
         assert n == 1 and ms > 0 and work == pytest.approx(2.0 * nbf * nbf * nk * naux), name
         assert d[(name, "k_syrk")][1] == 1
