@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="scf", choices=["scf", "jk", "batched"])
     ap.add_argument("--clusters", type=int, default=512, help="batched mode: number of water clusters")
+    ap.add_argument("--batch-prec", default="bf16x3", choices=["fp64", "fp32", "bf16x1", "bf16x2", "bf16x3"],
+                    help="batched mode: J/K precision of the batch")
+    ap.add_argument("--batch-budget-gib", type=float, default=150.0, help="batched mode: device memory per wave")
     ap.add_argument("--workload", default="h2o64", choices=["h2o64", "h2o16"])
     ap.add_argument("--model", default=str(ROOT / "rest_amd" / "models" / "learned_guess_h2o.npz"))
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x2", "bf16x1"],
@@ -358,52 +361,32 @@ def main_scf(args):
 
 
 def main_batched(args):
-    """Ragged J/K of many water clusters (n uniform in [2, 32], seeds splitmix64(base ^ i)) in grouped
-    persistent launches replayed from a hipGraph, FP32 B; clusters are packed into waves that fit HBM."""
+    """Ragged J/K of many water clusters ((H2O)n, n uniform in [2, 32], workload.batched_cluster_sizes)
+    in grouped persistent launches replayed from a hipGraph, in device-memory waves planned by the
+    library (rjk_batch_plan_waves); autotuned tiles per wave.  A step is one J and one K of every
+    cluster; the value is clusters per second over the summed replay times of all waves."""
     import torch
     from rest_amd import jk
     from rest_amd import workload as wl
 
-    base = 0x5EED
-    sizes = []
-    for i in range(args.clusters):
-        _, z = wl._splitmix64((base ^ i) & ((1 << 64) - 1))
-        sizes.append(2 + (z >> 11) * 31 // (1 << 53))
+    sizes = wl.batched_cluster_sizes(args.clusters)
     nbf = [24 * n for n in sizes]
     naux = [113 * n for n in sizes]
     nocc = [5 * n for n in sizes]
-
-    def dev_bytes(n):   # FP32 B + M' + FP64 SYRK partials, per cluster
-        nb, na, no = 24 * n, 113 * n, 5 * n
-        npair = nb * (nb + 1) // 2
-        ldb = -(-npair // 64) * 64
-        nbp = -(-nb // 128) * 128
-        nk = -(-(-(-no // 16) * 16) // 64) * 64
-        pps = max(1, 16384 // nk)
-        nsplit = -(-(-(-na // pps)) // 8) * 8
-        return 4 * ldb * na + 4 * na * nk * nbp + 8 * nsplit * nbp * nbp
-
-    budget = 150 * (1 << 30)
-    waves, cur, used = [], [], 0
-    for i, n in enumerate(sizes):
-        bsz = dev_bytes(n)
-        if cur and used + bsz > budget:
-            waves.append(cur)
-            cur, used = [], 0
-        cur.append(i)
-        used += bsz
-    if cur:
-        waves.append(cur)
+    prec = jk.PREC_NAMES[args.batch_prec]
+    waves = jk.batch_plan_waves(nbf, naux, nocc, prec, int(args.batch_budget_gib * (1 << 30)))
     stream = torch.cuda.Stream(device=torch.device("cuda:0"))
     torch.cuda.set_stream(stream)
-    t_total, tuned, kflop, bbytes = 0.0, [], 0.0, 0
-    for wi, idx in enumerate(waves):
-        b = jk.Batch([nbf[i] for i in idx], [naux[i] for i in idx], [nocc[i] for i in idx], jk.PREC_FP32)
+    es = 8 if prec == jk.PREC_FP64 else 4
+    t_total, tuned, stage_ms = 0.0, [], np.zeros(jk.BATCH_NUM_STAGES)
+    work = {"j": 0.0, "half": 0.0, "syrk": 0.0}
+    for wi, (i0, i1) in enumerate(waves):
+        b = jk.Batch(nbf[i0:i1], naux[i0:i1], nocc[i0:i1], prec)
         b.set_stream(stream.cuda_stream)
-        b.fill_synthetic(1000 * wi + 1)
-        for k, i in enumerate(idx):
-            c, _ = np.linalg.qr(np.random.default_rng(i + 1).standard_normal((nbf[i], nocc[i])))
-            b.set_inputs(k, 2.0 * c @ c.T, c, np.full(nocc[i], 2.0))
+        b.fill_synthetic(1000 + i0)
+        for i in range(i0, i1):
+            c = wl.random_orthonormal(i + 1, nbf[i], nocc[i])
+            b.set_inputs(i - i0, 2.0 * c @ c.T, c, np.full(nocc[i], 2.0))
         tuned.append((b.autotune(reps=1), b.config()))
         for _ in range(max(1, args.warmup)):
             b.run(1.0, -0.5)
@@ -416,29 +399,54 @@ def main_batched(args):
         ev1.record(stream)
         torch.cuda.synchronize()
         t_total += ev0.elapsed_time(ev1) / 1e3 / args.steps
-        for i in idx:
-            n_, o_ = nbf[i], nocc[i]
-            kflop += 2.0 * n_ * n_ * o_ * naux[i] + n_ * (n_ + 1.0) * o_ * naux[i]
-        bbytes += b.bytes_b
+        stage_ms += np.array(b.profile(1.0, -0.5))      # one staged launch sequence, events between stages
+        for i in range(i0, i1):
+            n_, o_, a_ = nbf[i], nocc[i], naux[i]
+            npair = n_ * (n_ + 1) // 2
+            work["j"] += 2.0 * npair * a_ * es + 8.0 * (n_ * n_ + npair) + 16.0 * a_
+            work["half"] += 2.0 * n_ * n_ * o_ * a_
+            work["syrk"] += n_ * (n_ + 1.0) * o_ * a_
         vj, vk = b.get_outputs(0)
         assert np.isfinite(vj).all() and np.isfinite(vk).all()
         b.close()
-        print(f"[bench] wave {wi + 1}/{len(waves)}: {len(idx)} clusters", file=sys.stderr, flush=True)
-    ach = kflop / t_total / 1e12
+        print(f"[bench] wave {wi + 1}/{len(waves)}: clusters {i0}..{i1 - 1}", file=sys.stderr, flush=True)
+    st = dict(zip(jk.BATCH_STAGES, stage_ms.tolist()))
+    j_ms = st["gamma"] + st["j_acc"]
+    half_ms = st["k_half_narrow"] + st["k_half_wide"]
+    syrk_ms = st["k_syrk_narrow"] + st["k_syrk_wide"]
+    kmult = BF16_PRODUCTS.get(args.batch_prec, 1)
+    kpeak = MFMA_PEAK_TFS["bf16" if args.batch_prec.startswith("bf16") else args.batch_prec]
+    cand = {"k_half": (half_ms, "mfma", work["half"]), "k_syrk": (syrk_ms, "mfma", work["syrk"]),
+            "j": (j_ms, "hbm", work["j"])}
+    dom = max(cand, key=lambda k: cand[k][0])
+    ms, bound, amount = cand[dom]
+    if bound == "hbm":
+        ach, peak, unit = amount / (ms / 1e3) / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        ach, peak, unit = kmult * amount / (ms / 1e3) / 1e12, kpeak, "TFLOP/s"
     out = {
-        "metric": f"batched RI-V J/K builds/s, {args.clusters} water clusters (H2O)n, n in [2, 32], def2-SVP/JKFIT shapes",
+        "metric": f"batched RI-V J/K builds/s, {args.clusters} water clusters (H2O)n, n in [2, 32], "
+                  f"def2-SVP/JKFIT shapes, {args.batch_prec}",
         "value": args.clusters / t_total, "unit": "J/K builds/s (one J + one K per cluster)", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t_total, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (B, MFMA) with f64 J and split-K sums",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 (B, MFMA) with f64 J and split-K sums"}.get(
+            args.batch_prec, f"{args.batch_prec}: B in f32, K on split-bf16 MFMA with f32 accumulation, J f32 B / f64 sums"),
         "data": "synthetic B (seeded hash generator), D = 2 C C^T from seeded orthonormal C",
         "config": {"workload": f"{args.clusters} clusters in {len(waves)} waves of grouped persistent launches "
                                "(hipGraph replay), autotuned tiles", "clusters": args.clusters,
-                   "waves": len(waves), "total_b_GB": bbytes / 1e9},
-        "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_PEAK_TFS["fp32"], "unit": "TFLOP/s",
-                     "frac": ach / MFMA_PEAK_TFS["fp32"], "traffic": None, "kernel": "whole grouped J/K (K FLOP)"},
+                   "waves": len(waves), "budget_gib": args.batch_budget_gib,
+                   "device_bytes_total": jk.batch_device_bytes(nbf, naux, nocc, prec)},
+        "roofline": {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
+                     "traffic": None, "kernel": f"{dom} stage (grouped, all waves)",
+                     "algorithmic_total": amount, "stage_ms_total": ms, "mfma_products_per_product": kmult},
         "cpu_baseline": None,
         "detail": {"autotune": [{"best_ms": t, "khalf_cut": c[0], "waves_per_cu": c[1]} for t, c in tuned],
-                   "j_pass_GBps_lower_bound": 2 * bbytes / t_total / 1e9},
+                   "stage_ms_summed_over_waves": st,
+                   "j_GBps": work["j"] / (j_ms / 1e3) / 1e9 if j_ms else None,
+                   "k_half_TFLOPs_algorithmic": work["half"] / (half_ms / 1e3) / 1e12 if half_ms else None,
+                   "k_syrk_TFLOPs_algorithmic": work["syrk"] / (syrk_ms / 1e3) / 1e12 if syrk_ms else None,
+                   "staged_ms_total": float(stage_ms.sum()), "graph_ms_total": 1e3 * t_total},
     }
     print(json.dumps(out))
 

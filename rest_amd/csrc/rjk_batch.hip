@@ -15,7 +15,17 @@
 //   γ_P   = Σ_p B_pP w_p,  J_p = s_J Σ_P B_pP γ_P      (vj_upper_with_rimatr_v_sync)
 //   M'_P  = B_P C',  C' = C diag(sqrt occ),  K = s_K Σ_P M'_P M'_Pᵀ (vk_upper_with_ri_v_sync)
 // with the K tiles of rjk_ktiles.hpp.  B is stored in the batch's precision
-// (FP64, or FP32 with FP64 J accumulation and FP32 MFMA K as in rjk_k.hip).
+// (FP64, or FP32 with FP64 J accumulation and FP32 MFMA K as in rjk_k.hip, FP64 partial
+// sums every kFp32Chain products).
+//
+// BF16 precisions (RJK_PREC_BF16X1/X2/X3): B is kept once, packed in FP32, and
+// J runs on it as at FP32.  The K half-transform reads the packed FP32 B and
+// splits every element into S BF16 planes while staging it into LDS (no plane
+// copy of B is stored), then runs the split products of rjk_kbf.hip on the
+// BF16 matrix cores; M' and the SYRK are those of rjk_kbf.hip per problem.  Both
+// stages keep the leading product (plane 0 × plane 0) and the correction
+// products in separate FP32 accumulators, added at the end, so the corrections'
+// accumulation rounding stays at their own scale.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +35,7 @@
 #include <vector>
 
 #include "../../include/rest_batch.h"
+#include "rjk_bftile.hpp"
 #include "rjk_ktiles.hpp"
 
 namespace rjk {
@@ -36,6 +47,7 @@ struct BProb {
     const void* b;        // packed B [naux][ldb], batch precision
     size_t ldb, npair;
     int nbf, naux, nbf_pad, nk, nocc, ncb, nsplit, pps;
+    int active;           // 0: every kernel skips this problem (its outputs keep their last values)
     const double* dm;     // [nbf][nbf] column-major
     const double* c;      // [nbf][nocc] column-major
     const double* occ;    // [nocc]
@@ -43,9 +55,9 @@ struct BProb {
     double* gamma;        // [naux]
     double* vj;           // [npair]
     double* vk;           // [npair]
-    void* cocc;           // [nbf_pad][nk]
-    void* sign;           // [nk]
-    void* m;              // [naux][nk][nbf_pad]
+    void* cocc;           // [nbf_pad][nk]; BF16: C'ᵀ planes [S][nk][nbf_pad]
+    void* sign;           // [nk]; BF16: sign masks (0x8000 for a negative weight) [nk]
+    void* m;              // [naux][nk][nbf_pad]; BF16: M' planes [S][naux][nbf_pad][nk]
     double* kpart;        // [nsplit][nbf_pad][nbf_pad]
 };
 
@@ -91,6 +103,7 @@ __global__ __launch_bounds__(256) void k_b_weights(const BProb* __restrict__ pr,
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const int q = find_prob(tb.pairs, np, t);
         const BProb& P = pr[q];
+        if (!P.active) continue;
         const size_t p = t - tb.pairs[q];
         size_t i, j;
         pair_ij(p, &i, &j);
@@ -114,6 +127,7 @@ __global__ __launch_bounds__(256) void k_b_gamma(const BProb* __restrict__ pr, i
     for (size_t it = blockIdx.x; it < total; it += gridDim.x) {
         const int q = find_prob(tb.cols, np, it);
         const BProb& P = pr[q];
+        if (!P.active) continue;
         const size_t col = it - tb.cols[q];
         const T* __restrict__ bc = (const T*)P.b + col * P.ldb;
         double acc = 0.0;
@@ -136,6 +150,7 @@ __global__ __launch_bounds__(256) void k_b_jacc(const BProb* __restrict__ pr, in
     for (size_t it = blockIdx.x; it < total; it += gridDim.x) {
         const int q = find_prob(tb.jseg, np, it);
         const BProb& P = pr[q];
+        if (!P.active) continue;
         const size_t p0 = (it - tb.jseg[q]) * (256 * NV) + (size_t)threadIdx.x * NV;
         if (p0 >= P.ldb) continue;
         double acc[NV];
@@ -173,6 +188,7 @@ __global__ __launch_bounds__(256) void k_b_prep(const BProb* __restrict__ pr, in
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const int q = find_prob(tb.prep, np, t);
         const BProb& P = pr[q];
+        if (!P.active) continue;
         const size_t e = t - tb.prep[q];
         const int c = (int)(e % P.nk), nu = (int)(e / P.nk);
         double v = 0.0;
@@ -195,6 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_b_khalf(const BProb* __restrict__ pr
     for (size_t it = blockIdx.x; it < total; it += gridDim.x) {
         const int j = find_prob(off, nsub, it);
         const BProb& P = pr[sub[j]];
+        if (!P.active) continue;
         const size_t local = it - off[j];
         const int ntm = P.nbf_pad / TM, ntc = P.nk / TN, per = ntm * ntc;
         const size_t xcd = local % kXcd, kq = local / kXcd;
@@ -213,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_b_ksyrk(const BProb* __restrict__ pr
     for (size_t it = blockIdx.x; it < total; it += gridDim.x) {
         const int q = find_prob(tb.syrk, np, it);
         const BProb& P = pr[q];
+        if (!P.active) continue;
         const size_t local = it - tb.syrk[q];
         const size_t nt = (size_t)P.nbf_pad / Tt, nt2 = nt * (nt + 1) / 2;
         const size_t xcd = local % kXcd, kq = local / kXcd;
@@ -231,6 +249,7 @@ __global__ __launch_bounds__(256) void k_b_kreduce(const BProb* __restrict__ pr,
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
         const int q = find_prob(tb.pairs, np, t);
         const BProb& P = pr[q];
+        if (!P.active) continue;
         const size_t p = t - tb.pairs[q];
         size_t i, j;
         pair_ij(p, &i, &j);
@@ -241,9 +260,349 @@ __global__ __launch_bounds__(256) void k_b_kreduce(const BProb* __restrict__ pr,
     }
 }
 
+// ---- K on split BF16 planes -----------------------------------------------------
+template <int S>
+__device__ inline void split_to(double x, __bf16* out) {
+    double r = x;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const __bf16 h = (__bf16)(float)r;
+        out[k] = h;
+        r -= (double)(float)h;
+    }
+}
+
+template <int S>
+__device__ inline void split_to_f(float x, __bf16* out) {
+    float r = x;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const __bf16 h = (__bf16)r;
+        out[k] = h;
+        r -= (float)h;
+    }
+}
+
+// C'ᵀ planes (column c of C diag(sqrt|occ|) as a row, ν fastest) and sign masks
+template <int S>
+__global__ __launch_bounds__(256) void k_b_bf_prep(const BProb* __restrict__ pr, int np, Tables tb) {
+    const size_t total = tb.prep[np];
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int q = find_prob(tb.prep, np, t);
+        const BProb& P = pr[q];
+        if (!P.active) continue;
+        const size_t e = t - tb.prep[q], plane = (size_t)P.nk * P.nbf_pad;
+        const int c = (int)(e / P.nbf_pad), nu = (int)(e % P.nbf_pad);
+        const double w = c < P.nocc ? P.occ[c] : 0.0;
+        const double v = (c < P.nocc && nu < P.nbf) ? P.c[(size_t)c * P.nbf + nu] * sqrt(fabs(w)) : 0.0;
+        __bf16 h[S];
+        split_to<S>(v, h);
+#pragma unroll
+        for (int k = 0; k < S; ++k) ((__bf16*)P.cocc)[k * plane + e] = h[k];
+        if (nu == 0) ((unsigned short*)P.sign)[c] = w < 0 ? 0x8000 : 0;
+    }
+}
+
+// Half-transform tile D[c][μ] = Σ_ν C'ᵀ[c][ν] B[μ][ν] of 64 c × TM μ on 2 × 2 waves, k-blocks of TK
+// ν.  B comes from the packed FP32 column (element (i<=j) at j(j+1)/2 + i); each thread stages
+// runs of 8 consecutive ν of one μ: along ν (contiguous in memory) when the tile lies wholly below
+// the diagonal, else one μ per lane (contiguous across lanes above the diagonal).  LDS rows are TK
+// BF16 values, 16-byte piece p of row r at p ^ sw(r) (the swizzle of rjk_bftile.hpp).
+template <int S, int TM, int TK>
+struct BfHalf {
+    static constexpr int NT = 256, WX = 2, WY = 2, WGX = 64, WGY = TM, NPC = TK / 8;
+    static constexpr int FX = WGX / WX / 16, FY = WGY / WY / 16;
+    static constexpr int LX = WGX * NPC / NT, LY = WGY * NPC / NT;
+    static constexpr int LDS_ELEMS = S * (WGX + WGY) * TK;
+    static constexpr int S_ = S, TK_ = TK;
+    static_assert(LX >= 1 && LY >= 1 && LX * NT == WGX * NPC && LY * NT == WGY * NPC, "load shape");
+    __device__ static constexpr int sw(int r) { return NPC == 4 ? (((r >> 1) ^ (r >> 3)) & 3) : ((r ^ (r >> 3)) & 7); }
+};
+
+template <class H>
+__device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const float* __restrict__ bcol, int nbf,
+                                             int mu0, int c0, const __bf16* __restrict__ ct, size_t pct,
+                                             int nbf_pad, int nk, __bf16* __restrict__ mP, size_t pm) {
+    using namespace bft;
+    constexpr int S = H::S_, NP = npairs<S>(), TK = H::TK_, NPC = H::NPC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wx = wave % H::WX, wy = wave / H::WX, l16 = lane & 15;
+    auto at = [](int r, int p) { return r * TK + ((p ^ H::sw(r)) << 3); };
+    auto Xs = [&](int k) { return lds + (size_t)k * (H::WGX + H::WGY) * TK; };
+    auto Ys = [&](int k) { return lds + ((size_t)k * (H::WGX + H::WGY) + H::WGX) * TK; };
+    f4 acc[H::FX][H::FY], acc2[H::FX][H::FY];   // leading product; corrections (added at the end)
+#pragma unroll
+    for (int i = 0; i < H::FX; ++i)
+#pragma unroll
+        for (int j = 0; j < H::FY; ++j) {
+            acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+            acc2[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    u4 rx[S][H::LX];
+    float rb[H::LY][8];
+    bool below = false;
+    auto task_rp = [&](int j, int* r, int* p) {   // row (μ offset) and piece of B task j
+        const int e = tid + j * H::NT;
+        if (below) {
+            *r = e / NPC;
+            *p = e % NPC;
+        } else {
+            *r = e % H::WGY;
+            *p = e / H::WGY;
+        }
+    };
+    auto fetch = [&](int nu0) {
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+#pragma unroll
+            for (int j = 0; j < H::LX; ++j) {
+                const int e = tid + j * H::NT;
+                rx[k][j] = *(const u4*)(ct + k * pct + (size_t)(c0 + e / NPC) * nbf_pad + nu0 + (e % NPC) * 8);
+            }
+        below = mu0 >= nu0 + TK - 1;
+#pragma unroll
+        for (int j = 0; j < H::LY; ++j) {
+            int r, p;
+            task_rp(j, &r, &p);
+            const int mu = mu0 + r, nub = nu0 + p * 8;
+            if (below) {
+                const size_t base = mu < nbf ? tri((size_t)mu) + nub : 0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) rb[j][i] = mu < nbf ? bcol[base + i] : 0.f;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int nu = nub + i;
+                    const bool ok = mu < nbf && nu < nbf;
+                    const int lo = mu < nu ? mu : nu, hi = mu < nu ? nu : mu;
+                    const float v = bcol[ok ? tri((size_t)hi) + lo : 0];
+                    rb[j][i] = ok ? v : 0.f;
+                }
+            }
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+#pragma unroll
+            for (int j = 0; j < H::LX; ++j) {
+                const int e = tid + j * H::NT;
+                *(u4*)(Xs(k) + at(e / NPC, e % NPC)) = rx[k][j];
+            }
+#pragma unroll
+        for (int j = 0; j < H::LY; ++j) {
+            int r, p;
+            task_rp(j, &r, &p);
+            bf8 v[S];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                __bf16 h[S];
+                split_to_f<S>(rb[j][i], h);
+#pragma unroll
+                for (int k = 0; k < S; ++k) v[k][i] = h[k];
+            }
+#pragma unroll
+            for (int k = 0; k < S; ++k) *(bf8*)(Ys(k) + at(r, p)) = v[k];
+        }
+    };
+    auto compute = [&]() {
+#pragma unroll
+        for (int ks = 0; ks < TK / KM; ++ks) {
+            const int pr = ks * 4 + (lane >> 4);
+            bf8 fx[H::FX][S], fy[H::FY][S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+#pragma unroll
+                for (int i = 0; i < H::FX; ++i) fx[i][k] = *(const bf8*)(Xs(k) + at((wx * H::FX + i) * 16 + l16, pr));
+#pragma unroll
+                for (int j = 0; j < H::FY; ++j) fy[j][k] = *(const bf8*)(Ys(k) + at((wy * H::FY + j) * 16 + l16, pr));
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int i = 0; i < H::FX; ++i)
+#pragma unroll
+                    for (int j = 0; j < H::FY; ++j) {
+                        if (p > 0)
+                            acc2[i][j] = mfma(fx[i][pa<S>(p)], fy[j][pb<S>(p)], acc2[i][j]);
+                        else
+                            acc[i][j] = mfma(fx[i][pa<S>(p)], fy[j][pb<S>(p)], acc[i][j]);
+                    }
+        }
+    };
+    const int nq = (nbf + TK - 1) / TK;
+    fetch(0);
+    for (int q = 0; q < nq; ++q) {
+        stash();
+        __syncthreads();
+        if (q + 1 < nq) fetch((q + 1) * TK);
+        compute();
+        __syncthreads();
+    }
+    // M' planes: lane holds c = c0 + 16(wx FX + i) + 4(l/16) + r for μ = mu0 + 16(wy FY + j) + l%16
+#pragma unroll
+    for (int i = 0; i < H::FX; ++i)
+#pragma unroll
+        for (int j = 0; j < H::FY; ++j) {
+            const int c = c0 + (wx * H::FX + i) * 16 + 4 * (lane >> 4);
+            const int mu = mu0 + (wy * H::FY + j) * 16 + l16;
+            __bf16 h[4][S];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) split_to_f<S>(acc[i][j][r] + acc2[i][j][r], h[r]);
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                *(bft::bf4*)(mP + (size_t)k * pm + (size_t)mu * nk + c) = bft::bf4{h[0][k], h[1][k], h[2][k], h[3][k]};
+        }
+}
+
+// half-transform over the problems of one tile class: round_up(naux, 8) · ntm · ntc items per problem
+template <class H>
+__global__ __launch_bounds__(256) void k_b_bf_khalf(const BProb* __restrict__ pr, const int* __restrict__ sub,
+                                                    const size_t* __restrict__ off, int nsub) {
+    __shared__ __bf16 lds[H::LDS_ELEMS];
+    const size_t total = off[nsub];
+    for (size_t it = blockIdx.x; it < total; it += gridDim.x) {
+        const int j = find_prob(off, nsub, it);
+        const BProb& P = pr[sub[j]];
+        if (!P.active) continue;
+        const size_t local = it - off[j];
+        const int ntm = P.nbf_pad / H::WGY, ntc = P.nk / H::WGX, per = ntm * ntc;
+        const size_t xcd = local % kXcd, kq = local / kXcd;
+        const int Pl = (int)((kq / per) * kXcd + xcd);
+        if (Pl >= P.naux) continue;   // padding items (uniform per work-group)
+        const int rem = (int)(kq % per), mt = rem / ntc, ctile = rem % ntc;
+        const size_t pm = (size_t)P.naux * P.nbf_pad * P.nk;
+        bf_half_tile<H>(lds, (const float*)P.b + (size_t)Pl * P.ldb, P.nbf, mt * H::WGY, ctile * H::WGX,
+                        (const __bf16*)P.cocc, (size_t)P.nk * P.nbf_pad, P.nbf_pad, P.nk,
+                        (__bf16*)P.m + (size_t)Pl * P.nbf_pad * P.nk, pm);
+    }
+}
+
+// SYRK over the problems of one tile class: nsplit · ntile2 items per problem (split s on XCD s % 8),
+// Kpart[split][ν][μ] = Σ_{P in split, c} sign_c M'_P[ν][c] M'_P[μ][c] on upper tile pairs
+template <class C>
+__global__ __launch_bounds__(C::NT) void k_b_bf_ksyrk(const BProb* __restrict__ pr, const int* __restrict__ sub,
+                                                      const size_t* __restrict__ off, int nsub) {
+    __shared__ __bf16 lds[C::LDS_ELEMS];
+    const size_t total = off[nsub];
+    for (size_t it = blockIdx.x; it < total; it += gridDim.x) {
+        const int jj = find_prob(off, nsub, it);
+        const BProb& P = pr[sub[jj]];
+        if (!P.active) continue;
+        const size_t local = it - off[jj];
+        const size_t nt = (size_t)P.nbf_pad / C::WGX, nt2 = nt * (nt + 1) / 2;
+        const size_t xcd = local % kXcd, kq = local / kXcd;
+        const size_t t = kq % nt2;
+        const int split = (int)((kq / nt2) * kXcd + xcd);
+        size_t tm, tn;
+        upper_tile(t, &tm, &tn);
+        const int mu0 = (int)tm * C::WGY, nu0 = (int)tn * C::WGX;
+        const int ncb = (P.nocc + C::TK - 1) / C::TK;
+        const int Pb = split * P.pps, Pe = min(P.naux, Pb + P.pps);
+        const int nq = (Pe > Pb ? Pe - Pb : 0) * ncb;
+        const size_t mplane = (size_t)P.naux * P.nbf_pad * P.nk, mPs = (size_t)P.nbf_pad * P.nk;
+        const __bf16* mpl = (const __bf16*)P.m;
+        const unsigned short* smask = (const unsigned short*)P.sign;
+        bft::f4 acc[C::FX][C::FY], acc2[C::FX][C::FY];   // leading product; corrections
+        bft::nt_tile2<C, true>(
+            lds, [&](int q) { return mpl + (Pb + q / ncb) * mPs + (size_t)nu0 * P.nk + (q % ncb) * C::TK; },
+            (size_t)P.nk, mplane,
+            [&](int q) { return mpl + (Pb + q / ncb) * mPs + (size_t)mu0 * P.nk + (q % ncb) * C::TK; },
+            (size_t)P.nk, mplane,
+            [&](int q, int pc) {
+                const unsigned short* sm = smask + (q % ncb) * C::TK + pc;
+                return bft::u4{(unsigned)sm[0] | ((unsigned)sm[1] << 16), (unsigned)sm[2] | ((unsigned)sm[3] << 16),
+                               (unsigned)sm[4] | ((unsigned)sm[5] << 16), (unsigned)sm[6] | ((unsigned)sm[7] << 16)};
+            },
+            nq, acc, acc2);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wx = wave % C::WX, wy = wave / C::WX;
+        double* __restrict__ kp = P.kpart + (size_t)split * P.nbf_pad * P.nbf_pad;
+#pragma unroll
+        for (int i = 0; i < C::FX; ++i)
+#pragma unroll
+            for (int j = 0; j < C::FY; ++j) {
+                const int nb = nu0 + (wx * C::FX + i) * 16 + 4 * (lane >> 4);
+                const int mu = mu0 + (wy * C::FY + j) * 16 + (lane & 15);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    kp[(size_t)(nb + r) * P.nbf_pad + mu] = (double)acc[i][j][r] + (double)acc2[i][j][r];
+            }
+    }
+}
+
+// B of one problem from a packed FP64 device array [naux][ld] into the batch slot (precision T)
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_from_dev(const double* __restrict__ src, size_t ld, size_t npair,
+                                                    size_t naux, size_t ldb, T* __restrict__ dst) {
+    const size_t total = naux * ldb;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t P = t / ldb, p = t % ldb;
+        dst[t] = p < npair ? (T)src[P * ld + p] : T(0);
+    }
+}
+
 size_t rup(size_t x, size_t a) { return (x + a - 1) / a * a; }
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+
+bool is_bf(int prec) { return prec == RJK_PREC_BF16X1 || prec == RJK_PREC_BF16X2 || prec == RJK_PREC_BF16X3; }
+int planes_of(int prec) { return prec == RJK_PREC_BF16X3 ? 3 : (prec == RJK_PREC_BF16X1 ? 1 : 2); }
+size_t elem_b(int prec) { return prec == RJK_PREC_FP64 ? 8 : 4; }
+
+// Products per SYRK accumulator before the FP64 split sum: kFp32Chain (as the single-problem K) where
+// the accumulator is FP32, 16384 at FP64 (there only the split count, i.e. the parallelism, follows).
+size_t batch_chain(int prec) { return prec == RJK_PREC_FP64 ? 16384 : kFp32Chain; }
+
+// Geometry of one problem at a precision (pointers left null).
+BProb geometry(size_t nbf, size_t naux, size_t nocc, int prec) {
+    BProb P = BProb{};
+    P.nbf = (int)nbf;
+    P.naux = (int)naux;
+    P.nocc = (int)nocc;
+    P.npair = nbf * (nbf + 1) / 2;
+    P.ldb = rup(P.npair, kLdAlign);
+    if (is_bf(prec)) {
+        P.nbf_pad = (int)rup(nbf, 64);     // the 64-wide BF16 tiles; 128-wide ones where nbf_pad % 128 == 0
+        P.ncb = 0;
+        P.nk = (int)rup(nocc, kBfColGran);
+    } else {
+        P.nbf_pad = (int)rup(nbf, kTileMN);
+        P.ncb = (int)(rup(nocc, kTileK) / kTileK);
+        P.nk = (int)rup((size_t)P.ncb * kTileK, kColPad);
+    }
+    P.pps = (int)std::max<size_t>(1, batch_chain(prec) / P.nk);
+    P.nsplit = (int)rup((naux + P.pps - 1) / P.pps, 8);
+    P.pps = (int)((naux + P.nsplit - 1) / P.nsplit);
+    P.active = 1;
+    return P;
+}
+
+// Device slots of one problem: {element bytes, element count} in allocation order (the order of the
+// BProb fields b, dm, c, occ, w, gamma, vj, vk, cocc, sign, m, kpart).
+constexpr int kNumSlots = 12;
+void slot_sizes(const BProb& P, int prec, size_t (&elem)[kNumSlots], size_t (&count)[kNumSlots]) {
+    const size_t es = elem_b(prec), S = is_bf(prec) ? (size_t)planes_of(prec) : 1;
+    const size_t ek = is_bf(prec) ? 2 : es;   // C', sign and M' elements
+    const size_t nb = P.nbf, pad = P.nbf_pad, nk = P.nk, na = P.naux;
+    const size_t e[kNumSlots] = {es, 8, 8, 8, 8, 8, 8, 8, ek, ek, ek, 8};
+    const size_t c[kNumSlots] = {P.ldb * na, nb * nb,      nb * P.nocc, (size_t)P.nocc, P.ldb, na,
+                                 P.npair,    P.npair,      S * pad * nk, nk,            S * na * nk * pad,
+                                 (size_t)P.nsplit * pad * pad};
+    for (int k = 0; k < kNumSlots; ++k) {
+        elem[k] = e[k];
+        count[k] = c[k];
+    }
+}
+
+size_t prob_bytes(const BProb& P, int prec) {
+    size_t e[kNumSlots], c[kNumSlots], tot = 0;
+    slot_sizes(P, prec, e, c);
+    for (int k = 0; k < kNumSlots; ++k) tot += rup(e[k] * c[k], 256);
+    return tot;
+}
+
+bool valid_prec(int prec) { return prec == RJK_PREC_FP64 || prec == RJK_PREC_FP32 || is_bf(prec); }
+
+bool valid_shape(size_t nbf, size_t naux, size_t nocc) {
+    return nbf != 0 && naux != 0 && nocc != 0 && nocc <= nbf && nbf <= 32768 && naux <= (1u << 24);
 }
 
 }  // namespace
@@ -257,19 +616,21 @@ struct rjk_batch {
     std::vector<BProb> h;            // host copy of the problem table
     BProb* d_prob = nullptr;
     size_t* d_tab = nullptr;         // the five prefix tables back to back
-    int* d_sub = nullptr;            // [2][nprob] problem lists of the two half-transform variants
-    size_t* d_suboff = nullptr;      // [2][nprob + 1]
+    int* d_sub = nullptr;            // [2][nprob] problem lists of the two tile classes
+    size_t* d_suboff = nullptr;      // [2][nprob + 1] half-transform item offsets per class
+    size_t* d_syrkoff = nullptr;     // [2][nprob + 1] SYRK item offsets per class (BF16 precisions)
     std::vector<size_t> t_pairs, t_cols, t_jseg, t_prep, t_syrk;
     std::vector<void*> allocs;
     double* scratch = nullptr;       // FP64 staging for uploads / the synthetic fill
     size_t scratch_elems = 0;
-    size_t bytes_b = 0;
+    size_t bytes_b = 0, bytes_total = 0;
     // launch configuration (autotuned)
-    int khalf_cut = 256;             // nbf_pad below this uses the 64×64 half-transform tile
+    int khalf_cut = 256;             // nbf_pad below this uses the narrow tile class
     int waves = 2;                   // persistent work-groups per CU for the tile kernels
     int ncu = 256;
     int t_sub_n[2] = {0, 0};
     size_t t_sub_items[2] = {0, 0};
+    size_t t_syrk_items[2] = {0, 0};
     double sj = 1.0, sk = 1.0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -306,39 +667,51 @@ Tables tables(const rjk_batch* b) {
     return t;
 }
 
-// half-transform subsets for the current cut: 0 = 64×64 tiles (small nbf), 1 = 128×64
+// Tile class of a problem under the current cut: 0 = narrow (FP: 64 × 64 half-transform tiles;
+// BF16: 64-wide half-transform and SYRK tiles), 1 = wide (128); BF16 needs nbf_pad % 128 == 0 for 1.
+int tile_class(const rjk_batch* b, const BProb& P) {
+    if (P.nbf_pad < b->khalf_cut) return 0;
+    return (is_bf(b->prec) && P.nbf_pad % 128) ? 0 : 1;
+}
+
 int build_subsets(rjk_batch* b) {
     const int np = (int)b->h.size();
+    const bool bf = is_bf(b->prec);
     std::vector<int> sub[2];
-    std::vector<size_t> off[2];
-    for (int v = 0; v < 2; ++v) off[v].push_back(0);
+    std::vector<size_t> off[2], soff[2];
+    for (int v = 0; v < 2; ++v) {
+        off[v].push_back(0);
+        soff[v].push_back(0);
+    }
     for (int q = 0; q < np; ++q) {
         const BProb& P = b->h[q];
-        const int v = P.nbf_pad < b->khalf_cut ? 0 : 1;
+        const int v = tile_class(b, P);
         const int TM = v == 0 ? 64 : 128, TN = 64;
         sub[v].push_back(q);
         off[v].push_back(off[v].back() + rup(P.naux, 8) * (size_t)(P.nbf_pad / TM) * (P.nk / TN));
+        const size_t nt = (size_t)P.nbf_pad / TM;
+        soff[v].push_back(soff[v].back() + (bf ? (size_t)P.nsplit * (nt * (nt + 1) / 2) : 0));
     }
     std::vector<int> hs(2 * np, 0);
-    std::vector<size_t> ho(2 * (np + 1), 0);
+    std::vector<size_t> ho(2 * (np + 1), 0), hso(2 * (np + 1), 0);
     for (int v = 0; v < 2; ++v) {
         std::copy(sub[v].begin(), sub[v].end(), hs.begin() + v * np);
         std::copy(off[v].begin(), off[v].end(), ho.begin() + v * (np + 1));
-        // entries past the subset repeat the last offset (never searched: nsub bounds the search)
+        std::copy(soff[v].begin(), soff[v].end(), hso.begin() + v * (np + 1));
+        // entries past the subset stay zero (never searched: nsub bounds the search)
     }
     b->t_sub_n[0] = (int)sub[0].size();
     b->t_sub_n[1] = (int)sub[1].size();
     BHIP(b, hipMemcpyAsync(b->d_sub, hs.data(), hs.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipMemcpyAsync(b->d_suboff, ho.data(), ho.size() * sizeof(size_t), hipMemcpyHostToDevice, b->stream));
-    b->t_sub_items[0] = off[0].back();
-    b->t_sub_items[1] = off[1].back();
+    BHIP(b, hipMemcpyAsync(b->d_syrkoff, hso.data(), hso.size() * sizeof(size_t), hipMemcpyHostToDevice, b->stream));
+    for (int v = 0; v < 2; ++v) {
+        b->t_sub_items[v] = off[v].back();
+        b->t_syrk_items[v] = soff[v].back();
+    }
     BHIP(b, hipStreamSynchronize(b->stream));   // host vectors die here
     return RJK_OK;
 }
-
-}  // namespace
-
-namespace {
 
 unsigned grid_elem(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
 unsigned grid_items(const rjk_batch* b, size_t items, int per_cu) {
@@ -346,30 +719,100 @@ unsigned grid_items(const rjk_batch* b, size_t items, int per_cu) {
     return (unsigned)rup(g, 8);
 }
 
+// Records ev[k] before stage k and ev[kStages] after the last (ev may be null).  Stages: 0 weights,
+// 1 γ, 2 J accumulation, 3 C' preparation, 4-5 half-transform (narrow, wide), 6-7 SYRK (narrow /
+// all for FP, wide), 8 split reduction.
 template <typename T>
-int enqueue_t(rjk_batch* b) {
+int enqueue_fp(rjk_batch* b, hipEvent_t* ev) {
     const int np = (int)b->h.size();
     const Tables tb = tables(b);
     const hipStream_t st = b->stream;
+    auto mark = [&](int k) {
+        if (ev) (void)hipEventRecord(ev[k], st);
+    };
+    mark(0);
     hipLaunchKernelGGL(k_b_weights, dim3(grid_elem(b->t_pairs.back())), dim3(256), 0, st, b->d_prob, np, tb);
+    mark(1);
     hipLaunchKernelGGL(k_b_gamma<T>, dim3(grid_items(b, b->t_cols.back(), 8)), dim3(256), 0, st, b->d_prob, np, tb);
+    mark(2);
     hipLaunchKernelGGL(k_b_jacc<T>, dim3(grid_items(b, b->t_jseg.back(), 8)), dim3(256), 0, st, b->d_prob, np, tb,
                        b->sj);
+    mark(3);
     hipLaunchKernelGGL(k_b_prep<T>, dim3(grid_elem(b->t_prep.back())), dim3(256), 0, st, b->d_prob, np, tb);
+    mark(4);
     if (b->t_sub_items[0])
         hipLaunchKernelGGL((k_b_khalf<T, 2, 2>), dim3(grid_items(b, b->t_sub_items[0], b->waves)), dim3(kBlock), 0,
                            st, b->d_prob, b->d_sub, b->d_suboff, b->t_sub_n[0]);
+    mark(5);
     if (b->t_sub_items[1])
         hipLaunchKernelGGL((k_b_khalf<T, 4, 2>), dim3(grid_items(b, b->t_sub_items[1], b->waves)), dim3(kBlock), 0,
                            st, b->d_prob, b->d_sub + np, b->d_suboff + (np + 1), b->t_sub_n[1]);
+    mark(6);
     hipLaunchKernelGGL((k_b_ksyrk<T, 4>), dim3(grid_items(b, b->t_syrk.back(), b->waves)), dim3(kBlock), 0, st,
                        b->d_prob, np, tb);
+    mark(7);
+    mark(8);
     hipLaunchKernelGGL(k_b_kreduce, dim3(grid_elem(b->t_pairs.back())), dim3(256), 0, st, b->d_prob, np, tb, b->sk);
+    mark(9);
     BHIP(b, hipGetLastError());
     return RJK_OK;
 }
 
-int enqueue(rjk_batch* b) { return b->prec == RJK_PREC_FP64 ? enqueue_t<double>(b) : enqueue_t<float>(b); }
+template <int S>
+int enqueue_bf(rjk_batch* b, hipEvent_t* ev) {
+    constexpr int TK = S == 3 ? 32 : 64;
+    using H0 = BfHalf<S, 64, TK>;
+    using H1 = BfHalf<S, 128, TK>;
+    using Y0 = bft::Cfg<S, 64, 64, 2, 2, false, TK>;
+    using Y1 = bft::Cfg<S, 128, 128, 2, 4, false, TK>;
+    const int np = (int)b->h.size();
+    const Tables tb = tables(b);
+    const hipStream_t st = b->stream;
+    auto mark = [&](int k) {
+        if (ev) (void)hipEventRecord(ev[k], st);
+    };
+    mark(0);
+    hipLaunchKernelGGL(k_b_weights, dim3(grid_elem(b->t_pairs.back())), dim3(256), 0, st, b->d_prob, np, tb);
+    mark(1);
+    hipLaunchKernelGGL(k_b_gamma<float>, dim3(grid_items(b, b->t_cols.back(), 8)), dim3(256), 0, st, b->d_prob, np,
+                       tb);
+    mark(2);
+    hipLaunchKernelGGL(k_b_jacc<float>, dim3(grid_items(b, b->t_jseg.back(), 8)), dim3(256), 0, st, b->d_prob, np,
+                       tb, b->sj);
+    mark(3);
+    hipLaunchKernelGGL(k_b_bf_prep<S>, dim3(grid_elem(b->t_prep.back())), dim3(256), 0, st, b->d_prob, np, tb);
+    mark(4);
+    if (b->t_sub_items[0])
+        hipLaunchKernelGGL(k_b_bf_khalf<H0>, dim3(grid_items(b, b->t_sub_items[0], b->waves)), dim3(256), 0, st,
+                           b->d_prob, b->d_sub, b->d_suboff, b->t_sub_n[0]);
+    mark(5);
+    if (b->t_sub_items[1])
+        hipLaunchKernelGGL(k_b_bf_khalf<H1>, dim3(grid_items(b, b->t_sub_items[1], b->waves)), dim3(256), 0, st,
+                           b->d_prob, b->d_sub + np, b->d_suboff + (np + 1), b->t_sub_n[1]);
+    mark(6);
+    if (b->t_syrk_items[0])
+        hipLaunchKernelGGL(k_b_bf_ksyrk<Y0>, dim3(grid_items(b, b->t_syrk_items[0], b->waves)), dim3(Y0::NT), 0, st,
+                           b->d_prob, b->d_sub, b->d_syrkoff, b->t_sub_n[0]);
+    mark(7);
+    if (b->t_syrk_items[1])
+        hipLaunchKernelGGL(k_b_bf_ksyrk<Y1>, dim3(grid_items(b, b->t_syrk_items[1], b->waves)), dim3(Y1::NT), 0, st,
+                           b->d_prob, b->d_sub + np, b->d_syrkoff + (np + 1), b->t_sub_n[1]);
+    mark(8);
+    hipLaunchKernelGGL(k_b_kreduce, dim3(grid_elem(b->t_pairs.back())), dim3(256), 0, st, b->d_prob, np, tb, b->sk);
+    mark(9);
+    BHIP(b, hipGetLastError());
+    return RJK_OK;
+}
+
+int enqueue(rjk_batch* b, hipEvent_t* ev = nullptr) {
+    switch (b->prec) {
+        case RJK_PREC_FP64: return enqueue_fp<double>(b, ev);
+        case RJK_PREC_FP32: return enqueue_fp<float>(b, ev);
+        case RJK_PREC_BF16X1: return enqueue_bf<1>(b, ev);
+        case RJK_PREC_BF16X3: return enqueue_bf<3>(b, ev);
+        default: return enqueue_bf<2>(b, ev);
+    }
+}
 
 int capture(rjk_batch* b) {
     drop_graph(b);
@@ -387,36 +830,55 @@ int capture(rjk_batch* b) {
     return RJK_OK;
 }
 
-template <typename F>
-int alloc_slots(rjk_batch* b, size_t elem, F count, std::vector<size_t>* offs, void** base) {
-    offs->clear();
-    size_t tot = 0;
-    for (const BProb& P : b->h) {
-        offs->push_back(tot);
-        tot += rup(count(P) * elem, 256);
-    }
-    void* p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(tot, 256)) != hipSuccess) {
-        (void)hipGetLastError();
-        return bfail(b, RJK_ERR_NOMEM, "rjk_batch_create: device allocation of " + std::to_string(tot) + " bytes failed");
-    }
-    b->allocs.push_back(p);
-    *base = p;
-    return RJK_OK;
-}
-
 }  // namespace
 
 extern "C" {
+
+size_t rjk_batch_device_bytes(size_t nprob, const size_t* nbf, const size_t* naux, const size_t* nocc, int prec) {
+    if (!nbf || !naux || !nocc || !valid_prec(prec)) return 0;
+    size_t tot = 0;
+    for (size_t i = 0; i < nprob; ++i) {
+        if (!valid_shape(nbf[i], naux[i], nocc[i])) return 0;
+        tot += prob_bytes(geometry(nbf[i], naux[i], nocc[i], prec), prec);
+    }
+    return tot;
+}
+
+int rjk_batch_plan_waves(size_t nprob, const size_t* nbf, const size_t* naux, const size_t* nocc, int prec,
+                         size_t budget_bytes, size_t* wave_start, size_t* nwaves) {
+    if (!nbf || !naux || !nocc || !wave_start || !nwaves || nprob == 0 || !valid_prec(prec)) return RJK_ERR_ARG;
+    for (size_t i = 0; i < nprob; ++i)
+        if (!valid_shape(nbf[i], naux[i], nocc[i])) return RJK_ERR_ARG;
+    if (budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            (void)hipGetLastError();
+            return RJK_ERR_HIP;
+        }
+        budget_bytes = fr / 10 * 9;
+    }
+    size_t w = 0, used = 0;
+    wave_start[0] = 0;
+    for (size_t i = 0; i < nprob; ++i) {
+        const size_t pb = prob_bytes(geometry(nbf[i], naux[i], nocc[i], prec), prec);
+        if (i > wave_start[w] && used + pb > budget_bytes) {
+            wave_start[++w] = i;
+            used = 0;
+        }
+        used += pb;
+    }
+    wave_start[w + 1] = nprob;
+    *nwaves = w + 1;
+    return RJK_OK;
+}
 
 int rjk_batch_create(int device, size_t nprob, const size_t* nbf, const size_t* naux, const size_t* nocc, int prec,
                      rjk_batch** out) {
     if (!out || !nbf || !naux || !nocc || nprob == 0 || nprob > (1u << 20)) return RJK_ERR_ARG;
     *out = nullptr;
-    if (prec != RJK_PREC_FP64 && prec != RJK_PREC_FP32) return RJK_ERR_ARG;
+    if (!valid_prec(prec)) return RJK_ERR_ARG;
     for (size_t i = 0; i < nprob; ++i)
-        if (nbf[i] == 0 || naux[i] == 0 || nocc[i] == 0 || nocc[i] > nbf[i] || nbf[i] > 32768 || naux[i] > (1u << 24))
-            return RJK_ERR_ARG;
+        if (!valid_shape(nbf[i], naux[i], nocc[i])) return RJK_ERR_ARG;
     rjk_batch* b = new rjk_batch();
     b->device = device;
     b->prec = prec;
@@ -429,50 +891,54 @@ int rjk_batch_create(int device, size_t nprob, const size_t* nbf, const size_t* 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         b->ncu = prop.multiProcessorCount;
-    const size_t es = prec == RJK_PREC_FP64 ? 8 : 4;
-    const size_t chain = 16384;   // products per FP32 (or FP64) SYRK accumulator before the FP64 partial sum
+    const size_t es = elem_b(prec);
     b->h.resize(nprob);
     size_t max_compact = 0, max_pad = 0;
     for (size_t i = 0; i < nprob; ++i) {
-        BProb& P = b->h[i];
-        P = BProb{};
-        P.nbf = (int)nbf[i];
-        P.naux = (int)naux[i];
-        P.nocc = (int)nocc[i];
-        P.npair = nbf[i] * (nbf[i] + 1) / 2;
-        P.ldb = rup(P.npair, kLdAlign);
-        P.nbf_pad = (int)rup(nbf[i], kTileMN);
-        P.ncb = (int)(rup(nocc[i], kTileK) / kTileK);
-        P.nk = (int)rup((size_t)P.ncb * kTileK, kColPad);
-        P.pps = (int)std::max<size_t>(1, chain / P.nk);
-        P.nsplit = (int)rup((naux[i] + P.pps - 1) / P.pps, 8);
-        P.pps = (int)((naux[i] + P.nsplit - 1) / P.nsplit);
+        b->h[i] = geometry(nbf[i], naux[i], nocc[i], prec);
+        const BProb& P = b->h[i];
         max_compact = std::max(max_compact, P.npair * naux[i]);
         max_pad = std::max(max_pad, P.ldb * naux[i]);
         b->bytes_b += P.ldb * naux[i] * es;
+        b->bytes_total += prob_bytes(P, prec);
     }
-    std::vector<size_t> o;
-    void* base = nullptr;
-    int rc = RJK_OK;
-#define SLOTS(elem, expr, field, type)                                                              \
-    rc = alloc_slots(b, elem, [&](const BProb& P) { return (size_t)(expr); }, &o, &base);          \
-    if (rc) return fail_free(rc);                                                                    \
-    for (size_t i = 0; i < nprob; ++i) b->h[i].field = (type)((char*)base + o[i]);
-    SLOTS(es, P.ldb * P.naux, b, const void*)
-    SLOTS(8, P.nbf * (size_t)P.nbf, dm, const double*)
-    SLOTS(8, P.nbf * (size_t)P.nocc, c, const double*)
-    SLOTS(8, P.nocc, occ, const double*)
-    SLOTS(8, P.ldb, w, double*)
-    SLOTS(8, P.naux, gamma, double*)
-    SLOTS(8, P.npair, vj, double*)
-    SLOTS(8, P.npair, vk, double*)
-    SLOTS(es, (size_t)P.nbf_pad * P.nk, cocc, void*)
-    SLOTS(es, P.nk, sign, void*)
-    SLOTS(es, (size_t)P.naux * P.nk * P.nbf_pad, m, void*)
-    SLOTS(8, (size_t)P.nsplit * P.nbf_pad * P.nbf_pad, kpart, double*)
-#undef SLOTS
+    // one allocation per slot kind; problem i's slot at the 256-byte aligned running offset
+    for (int k = 0; k < kNumSlots; ++k) {
+        std::vector<size_t> o;
+        size_t tot = 0;
+        for (const BProb& P : b->h) {
+            size_t e[kNumSlots], c[kNumSlots];
+            slot_sizes(P, prec, e, c);
+            o.push_back(tot);
+            tot += rup(e[k] * c[k], 256);
+        }
+        void* base = nullptr;
+        if (hipMalloc(&base, std::max<size_t>(tot, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            b->err = "rjk_batch_create: device allocation of " + std::to_string(tot) + " bytes failed";
+            return fail_free(RJK_ERR_NOMEM);
+        }
+        b->allocs.push_back(base);
+        for (size_t i = 0; i < nprob; ++i) {
+            void* a = (char*)base + o[i];
+            BProb& P = b->h[i];
+            switch (k) {
+                case 0: P.b = a; break;
+                case 1: P.dm = (const double*)a; break;
+                case 2: P.c = (const double*)a; break;
+                case 3: P.occ = (const double*)a; break;
+                case 4: P.w = (double*)a; break;
+                case 5: P.gamma = (double*)a; break;
+                case 6: P.vj = (double*)a; break;
+                case 7: P.vk = (double*)a; break;
+                case 8: P.cocc = a; break;
+                case 9: P.sign = a; break;
+                case 10: P.m = a; break;
+                default: P.kpart = (double*)a; break;
+            }
+        }
+    }
     // zero the padding of every input/weight slot once (graphs hold no memsets)
-    for (void* p : b->allocs) (void)p;
     for (const BProb& P : b->h) {
         if (hipMemsetAsync(P.w, 0, P.ldb * 8, b->stream) != hipSuccess ||
             hipMemsetAsync((void*)P.b, 0, P.ldb * P.naux * es, b->stream) != hipSuccess ||
@@ -499,7 +965,8 @@ int rjk_batch_create(int device, size_t nprob, const size_t* nbf, const size_t* 
     if (hipMalloc((void**)&b->d_prob, nprob * sizeof(BProb)) != hipSuccess ||
         hipMalloc((void**)&b->d_tab, tab.size() * sizeof(size_t)) != hipSuccess ||
         hipMalloc((void**)&b->d_sub, 2 * nprob * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&b->d_suboff, 2 * (nprob + 1) * sizeof(size_t)) != hipSuccess)
+        hipMalloc((void**)&b->d_suboff, 2 * (nprob + 1) * sizeof(size_t)) != hipSuccess ||
+        hipMalloc((void**)&b->d_syrkoff, 2 * (nprob + 1) * sizeof(size_t)) != hipSuccess)
         return fail_free(RJK_ERR_NOMEM);
     if (hipMemcpyAsync(b->d_prob, b->h.data(), nprob * sizeof(BProb), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
         hipMemcpyAsync(b->d_tab, tab.data(), tab.size() * sizeof(size_t), hipMemcpyHostToDevice, b->stream) !=
@@ -518,7 +985,8 @@ int rjk_batch_destroy(rjk_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
     for (void* p : b->allocs) (void)hipFree(p);
-    for (void* p : {(void*)b->d_prob, (void*)b->d_tab, (void*)b->d_sub, (void*)b->d_suboff, (void*)b->scratch})
+    for (void* p : {(void*)b->d_prob, (void*)b->d_tab, (void*)b->d_sub, (void*)b->d_suboff, (void*)b->d_syrkoff,
+                    (void*)b->scratch})
         if (p) (void)hipFree(p);
     if (b->own) (void)hipStreamDestroy(b->own);
     delete b;
@@ -572,6 +1040,34 @@ int rjk_batch_fill_synthetic(rjk_batch* b, uint64_t seed) {
     return RJK_OK;
 }
 
+int rjk_batch_set_rimatr_dev(rjk_batch* b, size_t i, const double* rimatr_dev, size_t ld) {
+    if (!b || !rimatr_dev || i >= b->h.size() || ld < b->h[i].npair)
+        return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_set_rimatr_dev: bad argument (ld must be >= npair)") : RJK_ERR_ARG;
+    const BProb& P = b->h[i];
+    BHIP(b, hipSetDevice(b->device));
+    const unsigned g = grid_elem(P.ldb * P.naux);
+    if (b->prec == RJK_PREC_FP64)
+        hipLaunchKernelGGL(k_b_from_dev<double>, dim3(g), dim3(256), 0, b->stream, rimatr_dev, ld, P.npair,
+                           (size_t)P.naux, P.ldb, (double*)P.b);
+    else
+        hipLaunchKernelGGL(k_b_from_dev<float>, dim3(g), dim3(256), 0, b->stream, rimatr_dev, ld, P.npair,
+                           (size_t)P.naux, P.ldb, (float*)P.b);
+    BHIP(b, hipGetLastError());
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return RJK_OK;
+}
+
+int rjk_batch_set_active(rjk_batch* b, size_t i, int active) {
+    if (!b || i >= b->h.size()) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_set_active: bad argument") : RJK_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    BProb& P = b->h[i];
+    P.active = active ? 1 : 0;
+    // stream-ordered after earlier runs; the captured graph reads the table at every replay
+    BHIP(b, hipMemcpyAsync(&b->d_prob[i].active, &P.active, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return RJK_OK;
+}
+
 int rjk_batch_set_inputs(rjk_batch* b, size_t i, const double* dm, const double* eigv, const double* occ) {
     if (!b || i >= b->h.size() || !dm || !eigv || !occ)
         return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_set_inputs: bad argument") : RJK_ERR_ARG;
@@ -605,6 +1101,29 @@ int rjk_batch_run(rjk_batch* b, double scale_j, double scale_k) {
     }
     BHIP(b, hipGraphLaunch(b->exec, b->stream));
     return RJK_OK;
+}
+
+int rjk_batch_profile(rjk_batch* b, double scale_j, double scale_k, double* stage_ms) {
+    if (!b || !stage_ms) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_profile: bad argument") : RJK_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    b->sj = scale_j;
+    b->sk = scale_k;
+    drop_graph(b);   // the next rjk_batch_run captures again with these scales
+    hipEvent_t ev[RJK_BATCH_NUM_STAGES + 1] = {};
+    int rc = RJK_OK;
+    for (int k = 0; k <= RJK_BATCH_NUM_STAGES && !rc; ++k)
+        if (hipEventCreate(&ev[k]) != hipSuccess) rc = bfail(b, RJK_ERR_HIP, "rjk_batch_profile: hipEventCreate");
+    if (!rc) rc = enqueue(b, ev);
+    if (!rc && hipEventSynchronize(ev[RJK_BATCH_NUM_STAGES]) != hipSuccess)
+        rc = bfail(b, RJK_ERR_HIP, "rjk_batch_profile: hipEventSynchronize");
+    for (int k = 0; k < RJK_BATCH_NUM_STAGES && !rc; ++k) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+        stage_ms[k] = ms;
+    }
+    for (int k = 0; k <= RJK_BATCH_NUM_STAGES; ++k)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+    return rc;
 }
 
 int rjk_batch_synchronize(rjk_batch* b) {

@@ -59,10 +59,13 @@ __device__ inline f4 mfma(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32
 // Runs nq k-blocks.  xaddr(q) / yaddr(q): pointer to row 0, plane 0, first element of
 // k-block q; ldx / ldy row strides, px / py plane strides (elements); xmask(q, col):
 // the 8 sign masks (0x8000 flips) of the X piece at k-block column col, as a u4 of
-// BF16 pairs.
-template <class C, class XA, class YA, class XM>
-__device__ __forceinline__ void nt_tile(__bf16* __restrict__ lds, XA xaddr, size_t ldx, size_t px, YA yaddr,
-                                        size_t ldy, size_t py, XM xmask, int nq, f4 (&acc)[C::FX][C::FY]) {
+// BF16 pairs.  With SPLIT the leading product (plane 0 × plane 0) accumulates in acc and
+// the correction products in acc2, so that FP32 rounding of the accumulators acts on the
+// corrections at their own (2^-8 smaller) scale; the caller adds the two in FP64.
+template <class C, bool SPLIT, class XA, class YA, class XM>
+__device__ __forceinline__ void nt_tile2(__bf16* __restrict__ lds, XA xaddr, size_t ldx, size_t px, YA yaddr,
+                                         size_t ldy, size_t py, XM xmask, int nq, f4 (&acc)[C::FX][C::FY],
+                                         f4 (&acc2)[C::FX][C::FY]) {
     constexpr int S = C::S, NP = npairs<S>(), TK = C::TK, NPC = C::NPC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wx = wave % C::WX, wy = wave / C::WX;
@@ -76,7 +79,10 @@ __device__ __forceinline__ void nt_tile(__bf16* __restrict__ lds, XA xaddr, size
 #pragma unroll
     for (int i = 0; i < C::FX; ++i)
 #pragma unroll
-        for (int j = 0; j < C::FY; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < C::FY; ++j) {
+            acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+            acc2[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        }
     u4 rx[S][C::LX], ry[S][C::LY], rm;
     auto fetch = [&](int q) {
         const __bf16* xb = xaddr(q);
@@ -119,7 +125,12 @@ __device__ __forceinline__ void nt_tile(__bf16* __restrict__ lds, XA xaddr, size
 #pragma unroll
                 for (int i = 0; i < C::FX; ++i)
 #pragma unroll
-                    for (int j = 0; j < C::FY; ++j) acc[i][j] = mfma(fx[i][pa<S>(p)], fy[j][pb<S>(p)], acc[i][j]);
+                    for (int j = 0; j < C::FY; ++j) {
+                        if (SPLIT && p > 0)
+                            acc2[i][j] = mfma(fx[i][pa<S>(p)], fy[j][pb<S>(p)], acc2[i][j]);
+                        else
+                            acc[i][j] = mfma(fx[i][pa<S>(p)], fy[j][pb<S>(p)], acc[i][j]);
+                    }
         }
     };
     if (nq <= 0) return;
@@ -143,6 +154,13 @@ __device__ __forceinline__ void nt_tile(__bf16* __restrict__ lds, XA xaddr, size
             __syncthreads();
         }
     }
+}
+
+// One accumulator for every product.
+template <class C, class XA, class YA, class XM>
+__device__ __forceinline__ void nt_tile(__bf16* __restrict__ lds, XA xaddr, size_t ldx, size_t px, YA yaddr,
+                                        size_t ldy, size_t py, XM xmask, int nq, f4 (&acc)[C::FX][C::FY]) {
+    nt_tile2<C, false>(lds, xaddr, ldx, px, yaddr, ldy, py, xmask, nq, acc, acc);
 }
 
 }  // namespace bft

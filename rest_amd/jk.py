@@ -58,7 +58,8 @@ EXPORTS = [
     "rjk_batch_create", "rjk_batch_destroy", "rjk_batch_last_error", "rjk_batch_size", "rjk_batch_bytes_b",
     "rjk_batch_set_stream", "rjk_batch_set_rimatr", "rjk_batch_fill_synthetic", "rjk_batch_set_inputs",
     "rjk_batch_get_outputs", "rjk_batch_run", "rjk_batch_synchronize", "rjk_batch_autotune", "rjk_batch_config",
-    "rjk_batch_set_config",
+    "rjk_batch_set_config", "rjk_batch_device_bytes", "rjk_batch_plan_waves", "rjk_batch_set_rimatr_dev",
+    "rjk_batch_profile", "rjk_batch_set_active",
     # include/rest_lg.h
     "rlg_create", "rlg_destroy", "rlg_last_error", "rlg_num_params", "rlg_set_stream", "rlg_set_params",
     "rlg_get_params", "rlg_forward", "rlg_loss_grad", "rlg_adam_step", "rlg_reset_optimizer", "rlg_gemm_bf16",
@@ -146,6 +147,11 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rjk_batch_autotune": (i, [vp, i, pd]),
         "rjk_batch_config": (i, [vp, C.POINTER(i), C.POINTER(i)]),
         "rjk_batch_set_config": (i, [vp, i, i]),
+        "rjk_batch_device_bytes": (sz, [sz, vp, vp, vp, i]),
+        "rjk_batch_plan_waves": (i, [sz, vp, vp, vp, i, sz, vp, C.POINTER(sz)]),
+        "rjk_batch_set_rimatr_dev": (i, [vp, sz, vp, sz]),
+        "rjk_batch_profile": (i, [vp, d, d, pd]),
+        "rjk_batch_set_active": (i, [vp, sz, i]),
         "rlg_create": (i, [i, sz, sz, sz, sz, sz, C.POINTER(vp)]),
         "rlg_destroy": (i, [vp]),
         "rlg_last_error": (C.c_char_p, [vp]),
@@ -344,6 +350,34 @@ class DeviceRI:
         self._check(self.lib.rjk_copy_dev(self.h, C.c_void_p(dst_ptr), C.c_void_p(src_ptr), int(nbytes)))
 
 
+BATCH_NUM_STAGES = 9
+BATCH_STAGES = ("weights", "gamma", "j_acc", "prep", "k_half_narrow", "k_half_wide", "k_syrk_narrow", "k_syrk_wide",
+                "k_reduce")
+
+
+def batch_device_bytes(nbf, naux, nocc, prec: int) -> int:
+    """Device bytes a Batch of these problems allocates (rjk_batch_device_bytes)."""
+    lib = load_library()
+    n = len(nbf)
+    arr = lambda v: (C.c_size_t * n)(*[int(x) for x in v])
+    return int(lib.rjk_batch_device_bytes(n, arr(nbf), arr(naux), arr(nocc), int(prec)))
+
+
+def batch_plan_waves(nbf, naux, nocc, prec: int, budget_bytes: int = 0) -> list:
+    """Consecutive index ranges [(start, stop), ...] whose batches fit budget_bytes of device memory
+    (rjk_batch_plan_waves; 0 = 90% of the free device memory)."""
+    lib = load_library()
+    n = len(nbf)
+    arr = lambda v: (C.c_size_t * n)(*[int(x) for x in v])
+    starts = (C.c_size_t * (n + 1))()
+    nw = C.c_size_t(0)
+    rc = lib.rjk_batch_plan_waves(n, arr(nbf), arr(naux), arr(nocc), int(prec), int(budget_bytes), starts,
+                                  C.byref(nw))
+    if rc != RJK_OK:
+        raise RJKError(rc, "rjk_batch_plan_waves failed")
+    return [(int(starts[w]), int(starts[w + 1])) for w in range(nw.value)]
+
+
 class Batch:
     """J/K of many problems in grouped persistent launches replayed from a hipGraph (include/rest_batch.h)."""
 
@@ -397,6 +431,24 @@ class Batch:
 
     def fill_synthetic(self, seed: int):
         self._check(self.lib.rjk_batch_fill_synthetic(self.h, C.c_uint64(seed)))
+
+    def set_rimatr_dev(self, i: int, ptr: int, ld: int):
+        """B of problem i from a device array [naux][ld] (padded rimatr layout)."""
+        self._check(self.lib.rjk_batch_set_rimatr_dev(self.h, i, C.c_void_p(ptr), int(ld)))
+
+    def set_from_ri(self, i: int, ri: "DeviceRI"):
+        """B of problem i copied on the device from a DeviceRI's FP64 storage."""
+        self.set_rimatr_dev(i, ri.rimatr_dev_ptr(), ri.ldb)
+
+    def set_active(self, i: int, active: bool):
+        """Include problem i in later runs or skip it (its outputs keep their last values)."""
+        self._check(self.lib.rjk_batch_set_active(self.h, i, 1 if active else 0))
+
+    def profile(self, scale_j: float = 1.0, scale_k: float = 1.0) -> list:
+        """Milliseconds of each of the BATCH_NUM_STAGES stages of one eager J/K of every problem."""
+        ms = (C.c_double * BATCH_NUM_STAGES)()
+        self._check(self.lib.rjk_batch_profile(self.h, float(scale_j), float(scale_k), ms))
+        return list(ms)
 
     def set_inputs(self, i: int, dm: np.ndarray, eigv: np.ndarray, occ: np.ndarray):
         d = np.ascontiguousarray(np.asarray(dm, dtype=np.float64).T)

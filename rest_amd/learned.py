@@ -721,6 +721,49 @@ def training_cluster(i: int, base_seed: int = 4096, nmin: int = 2, nmax: int = 2
     return wl.water_cluster(seed, n, oh_jitter=0.05, hoh_jitter_deg=5.0)
 
 
+def make_training_data_batched(indices, base_seed: int = 4096, basis="def2-SVP", auxbasis="def2-SVP-JKFIT",
+                               log=print, stats: dict = None, prec: int = None, budget_bytes: int = 0,
+                               chunk: int = 1024):
+    """Converge the training clusters with the batched SCF (batch_scf.rhf_batch: REST's loop per
+    cluster, every iteration's J/K of all clusters of a wave in one grouped launch, FP64 by default)
+    and encode the converged densities.  Clusters go through in chunks of `chunk` (host memory for
+    their one-electron matrices), each chunk in device-memory waves."""
+    from . import batch_scf as bs
+    prec = jk.PREC_FP64 if prec is None else prec
+    idx = list(indices)
+    Xs, Ts, Ms = [], [], []
+    st_all = {"clusters_done": 0, "not_converged": 0, "fallbacks": 0, "batch_runs": 0, "waves": 0,
+              "jk_seconds": 0.0, "host_seconds": 0.0, "setup_seconds": 0.0}
+    t0 = time.time()
+    for c0 in range(0, len(idx), chunk):
+        part = idx[c0:c0 + chunk]
+        mols = [ri.Molecule.water_cluster(training_cluster(i, base_seed), basis, auxbasis) for i in part]
+        st = {}
+        outs = bs.rhf_batch(mols, prec=prec, budget_bytes=budget_bytes, stats=st, log=log)
+        for k in ("fallbacks", "batch_runs", "waves", "jk_seconds", "host_seconds", "setup_seconds"):
+            st_all[k] += st.get(k, 0)
+        for i, mol, o in zip(part, mols, outs):
+            if not o.converged:
+                st_all["not_converged"] += 1
+                log(f"cluster {i}: SCF did not converge, skipped")
+                continue
+            da, db = gs.sad_density(mol)
+            X, T, M = encode(mol, o.dm, spherical_average(mol, da + db))
+            Xs.append(X)
+            Ts.append(T)
+            Ms.append(M)
+        st_all["clusters_done"] += len(part)
+        log(f"  {st_all['clusters_done']}/{len(idx)} clusters, {sum(len(x) for x in Xs)} pairs "
+            f"({time.time() - t0:.0f}s; J/K {st_all['jk_seconds']:.0f}s, host {st_all['host_seconds']:.0f}s, "
+            f"setup {st_all['setup_seconds']:.0f}s)")
+    if stats is not None:
+        stats.update(st_all)
+    if not Xs:
+        return (np.zeros((0, N_IN), np.float32), np.zeros((0, MAXB, MAXB), np.float32),
+                np.zeros((0, MAXB, MAXB), np.float32))
+    return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms)
+
+
 def make_training_data(indices, base_seed: int = 4096, basis="def2-SVP", auxbasis="def2-SVP-JKFIT", log=print,
                        deadline: float = None, stats: dict = None):
     """Run the FP64 SCF (SAD start) on each training cluster and encode its converged density.

@@ -7,12 +7,13 @@ it with SAD on held-out water clusters.
 
 Training clusters: learned.training_cluster(i), i < clusters: (H2O)n with n
 uniform in [2, 24], monomer O–H ±0.05 Å and H–O–H ±5°, def2-SVP /
-def2-SVP-JKFIT, converged with the FP64 device SCF from the SAD guess.  With
---workers W the clusters are split into W contiguous ranges, each converged by
-its own process on the same GPU (the per-cluster work — one-electron
-integrals on the host, the B build, small launch-bound SCF kernels — leaves
-the GPU mostly idle in a single process); every worker stops at the
-generation deadline and the training uses the clusters done by then.
+def2-SVP-JKFIT, converged from the SAD guess.  The default generator
+(--generator batched) is the batched SCF (batch_scf.rhf_batch): REST's loop
+per cluster with the FP64 J/K of all clusters of a memory wave in one grouped
+launch per iteration, replayed from a hipGraph.  --generator single converges
+one cluster at a time with the single-problem device SCF; with --workers W the
+clusters are then split into W contiguous ranges, each converged by its own
+process on the same GPU, and every worker stops at the generation deadline.
 Held-out test: (H2O)n clusters from workload.water_cluster with seeds not used
 in training (--test-n, --test-seeds); for each, the FP64 SCF from SAD and the
 SCF from the learned guess (learned.learned_scf with the --guard acceptance
@@ -148,7 +149,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--clusters", type=int, default=256)
     ap.add_argument("--base-seed", type=int, default=4096)
-    ap.add_argument("--workers", type=int, default=1, help="data-generation processes sharing the GPU")
+    ap.add_argument("--generator", default="batched", choices=["batched", "single"],
+                    help="training SCFs through the batched grouped J/K, or one cluster at a time")
+    ap.add_argument("--workers", type=int, default=1, help="single generator: processes sharing the GPU")
     ap.add_argument("--cpu-threads", type=int, default=16, help="host threads split among the workers")
     ap.add_argument("--gen-seconds", type=float, default=1e9, help="data-generation deadline (seconds)")
     ap.add_argument("--epochs", type=int, default=40)
@@ -196,8 +199,13 @@ def main(argv=None):
     if a.out is None:
         ap.error("--out is required for training")
     t0 = time.time()
-    log(f"generating training data: {a.clusters} clusters, {a.workers} worker(s)")
-    if a.workers > 1:
+    log(f"generating training data: {a.clusters} clusters, generator {a.generator}")
+    gen_stats = {}
+    if a.generator == "batched":
+        from . import learned as lg
+        X, T, M = lg.make_training_data_batched(range(a.clusters), a.base_seed, log=log, stats=gen_stats)
+        done = gen_stats["clusters_done"]
+    elif a.workers > 1:
         X, T, M, done = _generate_parallel(a)
     else:
         from . import learned as lg
@@ -211,7 +219,8 @@ def main(argv=None):
     net = lg.Net(lg.N_IN, lg.N_OUT, a.hidden, a.layers, max_batch=a.batch, seed=0)
     hist = lg.train(net, X, T, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
     meta = {"clusters": int(done), "clusters_requested": a.clusters, "base_seed": a.base_seed,
-            "samples": int(len(X)), "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "workers": a.workers,
+            "samples": int(len(X)), "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "generator": a.generator,
+            "workers": a.workers, "generation_stats": gen_stats,
             "train_loss": hist["train"][-1], "val_loss": hist["val"][-1], "generation_seconds": t_gen,
             "train_seconds": time.time() - t0 - t_gen}
     a.out.parent.mkdir(parents=True, exist_ok=True)

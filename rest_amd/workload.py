@@ -104,6 +104,16 @@ def water_cluster(seed: int, n: int, oh_jitter: float = 0.0, hoh_jitter_deg: flo
     raise RuntimeError("water cluster generator exhausted its attempts")
 
 
+def batched_cluster_sizes(count: int, base: int = 0x5EED, lo: int = 2, hi: int = 32) -> list:
+    """Water counts of the batched workload: cluster i has n = lo + floor(u (hi - lo + 1)), u the top
+    53 bits of splitmix64(base ^ i) as a fraction (n uniform in [lo, hi])."""
+    out = []
+    for i in range(count):
+        _, z = _splitmix64((base ^ i) & ((1 << 64) - 1))
+        out.append(lo + (z >> 11) * (hi - lo + 1) // (1 << 53))
+    return out
+
+
 def random_orthonormal(seed: int, nbf: int, nocc: int) -> np.ndarray:
     """Seeded Gaussian nbf×nocc (column-major draw order), orthonormalised by
     Householder QR with diag(R) >= 0."""
