@@ -49,6 +49,8 @@ typedef struct rjk_batch rjk_batch;
 /* Device bytes rjk_batch_create allocates for these problems at precision prec (B, inputs,
  * outputs and intermediates; 0 for invalid arguments).  No device work. */
 size_t rjk_batch_device_bytes(size_t nprob, const size_t* nbf, const size_t* naux, const size_t* nocc, int prec);
+/* Free and total bytes of device memory now (hipMemGetInfo on `device`). */
+int rjk_batch_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 /* Waves under a device-memory budget: problems 0..nprob-1, in the given order, split into
  * consecutive groups whose rjk_batch_device_bytes stays within budget_bytes (a problem larger than
  * the budget gets a wave of its own).  wave_start (nprob + 1 entries) receives the first problem

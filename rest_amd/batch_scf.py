@@ -24,6 +24,7 @@ Molecules go through in waves whose device memory fits a budget
 """
 from __future__ import annotations
 
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -143,38 +144,75 @@ def _rhf_loop(S, H, enuc: float, nocc: int, c_guess: np.ndarray, ctrl):
             return E, converged, num_iter - 1, D, C, eps, energies
 
 
+def _single_bytes(nbf: int, naux: int) -> int:
+    """Device bytes held by one molecule's set-up (its FP64 B and the guess step's work space, bounded
+    by twice B)."""
+    ldb = -(-(nbf * (nbf + 1) // 2) // 64) * 64
+    return 2 * 8 * ldb * naux
+
+
 def rhf_batch(mols, ctrl=None, prec: int = jk.PREC_FP64, budget_bytes: int = 0, threads: int = 16,
-              log=None, stats: dict = None) -> List[BatchOutcome]:
+              setup_threads: int = 8, log=None, stats: dict = None) -> List[BatchOutcome]:
     """Converge every molecule of `mols` (ri.Molecule, closed shell); returns their outcomes in order.
-    prec: the batch's J/K precision; budget_bytes: device memory per wave (0 = 90% of free memory)."""
+    prec: the batch's J/K precision; budget_bytes: device memory per wave (0: 90% of the free memory
+    less what `setup_threads` concurrent set-ups hold); threads: host threads for the per-molecule
+    loops; setup_threads: molecules whose integrals, B and SAD guess step are made at once."""
     from threadpoolctl import threadpool_limits
 
     ctrl = ctrl or rs.default_ctrl()
     nbf = [m.nbf for m in mols]
     naux = [m.naux for m in mols]
     nocc = [m.nelectron // 2 for m in mols]
+    if budget_bytes == 0:
+        free, _ = jk.device_memory()
+        reserve = setup_threads * max(_single_bytes(a, b) for a, b in zip(nbf, naux)) + (2 << 30)
+        budget_bytes = max(1 << 30, int(0.9 * free) - reserve)
     waves = jk.batch_plan_waves(nbf, naux, nocc, prec, budget_bytes)
+    for el in sorted({e for m in mols for e in m.symbols}):    # atomic densities cached before threads start
+        gs.atomic_density(el, mols[0].basis, mols[0].auxbasis)
     out: List[Optional[BatchOutcome]] = [None] * len(mols)
     st = {"waves": len(waves), "batch_runs": 0, "jk_seconds": 0.0, "host_seconds": 0.0, "setup_seconds": 0.0,
-          "fallbacks": 0}
+          "fallbacks": 0, "alloc_seconds": 0.0, "setup_int1e_thread_s": 0.0, "setup_b_thread_s": 0.0,
+          "setup_guess_thread_s": 0.0}
+    tlock = threading.Lock()
+
+    def tadd(key, v):
+        with tlock:
+            st[key] += v
     pool = ThreadPoolExecutor(max_workers=max(1, threads))
     store = jk.STORE_FP64
     try:
         for w, (i0, i1) in enumerate(waves):
             t0 = time.perf_counter()
             batch = jk.Batch(nbf[i0:i1], naux[i0:i1], nocc[i0:i1], prec)
+            st["alloc_seconds"] += time.perf_counter() - t0
             loops, pend = {}, {}
-            for k, i in enumerate(range(i0, i1)):
+
+            def setup(k):
+                i = i0 + k
                 m = mols[i]
+                ta = time.perf_counter()
                 S, T, V = m.int1e()
                 H = T + V
+                tb = time.perf_counter()
                 with jk.DeviceRI(m.nbf, m.naux) as r:
                     m.build_rimatr(r, 1e-8, store)
+                    tc = time.perf_counter()
                     c1 = gs.sad_guess(m, r, S, H).c1
                     batch.set_from_ri(k, r)
+                td = time.perf_counter()
+                tadd("setup_int1e_thread_s", tb - ta)
+                tadd("setup_b_thread_s", tc - tb)
+                tadd("setup_guess_thread_s", td - tc)
                 c1 = c1[0] if c1.ndim == 3 else c1
-                loops[k] = _rhf_loop(S, H, m.nuclear_repulsion(), nocc[i], c1, ctrl)
-                pend[k] = next(loops[k])
+                loop = _rhf_loop(S, H, m.nuclear_repulsion(), nocc[i], c1, ctrl)
+                return k, loop, next(loop)
+
+            with ThreadPoolExecutor(max_workers=max(1, setup_threads)) as ex, \
+                    threadpool_limits(limits=max(1, threads // max(1, setup_threads))):
+                for k, loop, req in ex.map(setup, range(i1 - i0)):
+                    loops[k] = loop
+                    pend[k] = req
             st["setup_seconds"] += time.perf_counter() - t0
             occs = {k: np.full(nocc[i0 + k], 2.0) for k in loops}
 
@@ -214,7 +252,9 @@ def rhf_batch(mols, ctrl=None, prec: int = jk.PREC_FP64, budget_bytes: int = 0, 
                         E, conv, it, D, C, eps, energies = fin
                         out[i] = BatchOutcome(E, bool(conv), it, D, C, eps, True, energies)
                 pend = new
+            t3 = time.perf_counter()
             batch.close()
+            st["alloc_seconds"] += time.perf_counter() - t3
             if log:
                 log(f"wave {w + 1}/{len(waves)}: molecules {i0}..{i1 - 1} done, {st['batch_runs']} batch runs so far")
     finally:

@@ -844,6 +844,22 @@ size_t rjk_batch_device_bytes(size_t nprob, const size_t* nbf, const size_t* nau
     return tot;
 }
 
+int rjk_batch_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return RJK_ERR_ARG;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(device) != hipSuccess) {
+        (void)hipGetLastError();
+        return RJK_ERR_HIP;
+    }
+    const hipError_t e = hipMemGetInfo(free_bytes, total_bytes);
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return RJK_ERR_HIP;
+    }
+    return RJK_OK;
+}
+
 int rjk_batch_plan_waves(size_t nprob, const size_t* nbf, const size_t* naux, const size_t* nocc, int prec,
                          size_t budget_bytes, size_t* wave_start, size_t* nwaves) {
     if (!nbf || !naux || !nocc || !wave_start || !nwaves || nprob == 0 || !valid_prec(prec)) return RJK_ERR_ARG;

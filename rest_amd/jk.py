@@ -59,7 +59,7 @@ EXPORTS = [
     "rjk_batch_set_stream", "rjk_batch_set_rimatr", "rjk_batch_fill_synthetic", "rjk_batch_set_inputs",
     "rjk_batch_get_outputs", "rjk_batch_run", "rjk_batch_synchronize", "rjk_batch_autotune", "rjk_batch_config",
     "rjk_batch_set_config", "rjk_batch_device_bytes", "rjk_batch_plan_waves", "rjk_batch_set_rimatr_dev",
-    "rjk_batch_profile", "rjk_batch_set_active",
+    "rjk_batch_profile", "rjk_batch_set_active", "rjk_batch_device_memory",
     # include/rest_lg.h
     "rlg_create", "rlg_destroy", "rlg_last_error", "rlg_num_params", "rlg_set_stream", "rlg_set_params",
     "rlg_get_params", "rlg_forward", "rlg_loss_grad", "rlg_adam_step", "rlg_reset_optimizer", "rlg_gemm_bf16",
@@ -152,6 +152,7 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         "rjk_batch_set_rimatr_dev": (i, [vp, sz, vp, sz]),
         "rjk_batch_profile": (i, [vp, d, d, pd]),
         "rjk_batch_set_active": (i, [vp, sz, i]),
+        "rjk_batch_device_memory": (i, [i, C.POINTER(sz), C.POINTER(sz)]),
         "rlg_create": (i, [i, sz, sz, sz, sz, sz, C.POINTER(vp)]),
         "rlg_destroy": (i, [vp]),
         "rlg_last_error": (C.c_char_p, [vp]),
@@ -361,6 +362,16 @@ def batch_device_bytes(nbf, naux, nocc, prec: int) -> int:
     n = len(nbf)
     arr = lambda v: (C.c_size_t * n)(*[int(x) for x in v])
     return int(lib.rjk_batch_device_bytes(n, arr(nbf), arr(naux), arr(nocc), int(prec)))
+
+
+def device_memory(device: int = 0) -> tuple:
+    """(free, total) bytes of device memory now."""
+    lib = load_library()
+    fr, tot = C.c_size_t(0), C.c_size_t(0)
+    rc = lib.rjk_batch_device_memory(int(device), C.byref(fr), C.byref(tot))
+    if rc != RJK_OK:
+        raise RJKError(rc, "rjk_batch_device_memory failed")
+    return fr.value, tot.value
 
 
 def batch_plan_waves(nbf, naux, nocc, prec: int, budget_bytes: int = 0) -> list:

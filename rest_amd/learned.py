@@ -723,11 +723,12 @@ def training_cluster(i: int, base_seed: int = 4096, nmin: int = 2, nmax: int = 2
 
 def make_training_data_batched(indices, base_seed: int = 4096, basis="def2-SVP", auxbasis="def2-SVP-JKFIT",
                                log=print, stats: dict = None, prec: int = None, budget_bytes: int = 0,
-                               chunk: int = 1024):
+                               chunk: int = 256, setup_threads: int = 16, deadline: float = None):
     """Converge the training clusters with the batched SCF (batch_scf.rhf_batch: REST's loop per
     cluster, every iteration's J/K of all clusters of a wave in one grouped launch, FP64 by default)
     and encode the converged densities.  Clusters go through in chunks of `chunk` (host memory for
-    their one-electron matrices), each chunk in device-memory waves."""
+    their one-electron matrices), each chunk in device-memory waves; no chunk starts after
+    `deadline` (a time.time() value)."""
     from . import batch_scf as bs
     prec = jk.PREC_FP64 if prec is None else prec
     idx = list(indices)
@@ -736,12 +737,16 @@ def make_training_data_batched(indices, base_seed: int = 4096, basis="def2-SVP",
               "jk_seconds": 0.0, "host_seconds": 0.0, "setup_seconds": 0.0}
     t0 = time.time()
     for c0 in range(0, len(idx), chunk):
+        if deadline is not None and time.time() > deadline:
+            log(f"deadline reached after {c0} clusters")
+            break
         part = idx[c0:c0 + chunk]
         mols = [ri.Molecule.water_cluster(training_cluster(i, base_seed), basis, auxbasis) for i in part]
         st = {}
-        outs = bs.rhf_batch(mols, prec=prec, budget_bytes=budget_bytes, stats=st, log=log)
-        for k in ("fallbacks", "batch_runs", "waves", "jk_seconds", "host_seconds", "setup_seconds"):
-            st_all[k] += st.get(k, 0)
+        outs = bs.rhf_batch(mols, prec=prec, budget_bytes=budget_bytes, setup_threads=setup_threads, stats=st,
+                            log=log)
+        for k, v in st.items():
+            st_all[k] = st_all.get(k, 0) + v
         for i, mol, o in zip(part, mols, outs):
             if not o.converged:
                 st_all["not_converged"] += 1

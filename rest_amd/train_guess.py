@@ -1,7 +1,8 @@
 """Train the learned initial guess on densities this SCF produces, then compare
 it with SAD on held-out water clusters.
 
-    python -m rest_amd.train_guess --clusters 4096 --workers 6 --epochs 60 --out rest_amd/models/learned_guess_h2o.npz
+    python -m rest_amd.train_guess --clusters 4096 --workers 4 --budget-gib 45 --setup-threads 2 --epochs 60 \
+        --out rest_amd/models/learned_guess_h2o.npz
     python -m rest_amd.train_guess --eval-only --model rest_amd/models/learned_guess_h2o.npz \\
         --test-n 64 --test-seeds 100 101 ... --report test.json
 
@@ -11,9 +12,14 @@ def2-SVP-JKFIT, converged from the SAD guess.  The default generator
 (--generator batched) is the batched SCF (batch_scf.rhf_batch): REST's loop
 per cluster with the FP64 J/K of all clusters of a memory wave in one grouped
 launch per iteration, replayed from a hipGraph.  --generator single converges
-one cluster at a time with the single-problem device SCF; with --workers W the
-clusters are then split into W contiguous ranges, each converged by its own
-process on the same GPU, and every worker stops at the generation deadline.
+one cluster at a time with the single-problem device SCF.  With --workers W the
+clusters are split into W contiguous ranges, each converged by its own process
+on the same GPU (the per-cluster set-up — one-electron integrals on the host,
+the B build, the SAD guess step — runs in every process at once; each batched
+worker keeps its waves within --budget-gib).  Generation stops at the
+--gen-seconds deadline (before the next chunk of clusters, or, for a
+single-generator worker, the next cluster) and the training uses the clusters
+done by then.
 Held-out test: (H2O)n clusters from workload.water_cluster with seeds not used
 in training (--test-n, --test-seeds); for each, the FP64 SCF from SAD and the
 SCF from the learned guess (learned.learned_scf with the --guard acceptance
@@ -105,13 +111,22 @@ def summarize(rows: list) -> dict:
 
 
 def _generate_shard(a) -> int:
-    """Worker: converge clusters [i0, i1) until the deadline, save (X, T, M) to --shard-out."""
+    """Worker: converge clusters [i0, i1) (single generator: until the deadline), save (X, T, M) and the
+    generation statistics to --shard-out."""
+    from . import jk
     from . import learned as lg
     i0, i1 = a.shard
     stats = {}
-    X, T, M = lg.make_training_data(range(i0, i1), a.base_seed, log=lambda m: log(f"[worker {i0}] {m}"),
-                                    deadline=a.deadline, stats=stats)
-    np.savez(a.shard_out, X=X, T=T, M=M.astype(np.uint8), done=np.int64(stats.get("clusters_done", 0)))
+    say = lambda m: log(f"[worker {i0}] {m}")
+    if a.generator == "batched":
+        X, T, M = lg.make_training_data_batched(range(i0, i1), a.base_seed, log=say, stats=stats,
+                                                prec=jk.PREC_NAMES[a.gen_prec],
+                                                budget_bytes=int(a.budget_gib * (1 << 30)),
+                                                setup_threads=a.setup_threads, deadline=a.deadline)
+    else:
+        X, T, M = lg.make_training_data(range(i0, i1), a.base_seed, log=say, deadline=a.deadline, stats=stats)
+    np.savez(a.shard_out, X=X, T=T, M=M.astype(np.uint8), done=np.int64(stats.get("clusters_done", 0)),
+             stats=json.dumps(stats))
     log(f"[worker {i0}] saved {stats.get('clusters_done', 0)} clusters, {len(X)} pairs")
     return 0
 
@@ -128,9 +143,10 @@ def _generate_parallel(a):
         out = tmp / f"shard{w}.npz"
         cmd = [sys.executable, "-u", "-m", "rest_amd.train_guess", "--shard", str(edges[w]), str(edges[w + 1]),
                "--shard-out", str(out), "--deadline", repr(deadline), "--base-seed", str(a.base_seed),
-               "--out", str(out)]
+               "--out", str(out), "--generator", a.generator, "--gen-prec", a.gen_prec,
+               "--budget-gib", repr(a.budget_gib), "--setup-threads", str(a.setup_threads)]
         procs.append((subprocess.Popen(cmd, env=env), out))
-    Xs, Ts, Ms, done = [], [], [], 0
+    Xs, Ts, Ms, done, stats = [], [], [], 0, {}
     for p, out in procs:
         rc = p.wait()
         if rc != 0:
@@ -140,9 +156,11 @@ def _generate_parallel(a):
             Ts.append(z["T"])
             Ms.append(z["M"].astype(np.float32))
             done += int(z["done"])
+            for k, v in json.loads(str(z["stats"])).items():
+                stats[k] = stats.get(k, 0) + v if isinstance(v, (int, float)) else v
         out.unlink()
     tmp.rmdir()
-    return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms), done
+    return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms), done, stats
 
 
 def main(argv=None):
@@ -151,7 +169,13 @@ def main(argv=None):
     ap.add_argument("--base-seed", type=int, default=4096)
     ap.add_argument("--generator", default="batched", choices=["batched", "single"],
                     help="training SCFs through the batched grouped J/K, or one cluster at a time")
-    ap.add_argument("--workers", type=int, default=1, help="single generator: processes sharing the GPU")
+    ap.add_argument("--gen-prec", default="fp64", choices=["fp64", "fp32", "bf16x3"],
+                    help="batched generator: J/K precision of the training SCFs")
+    ap.add_argument("--workers", type=int, default=1, help="data-generation processes sharing the GPU")
+    ap.add_argument("--budget-gib", type=float, default=0.0,
+                    help="batched generator: device memory per batch wave in each process (0: from free memory)")
+    ap.add_argument("--setup-threads", type=int, default=16,
+                    help="batched generator: clusters whose integrals and B are built at once per process")
     ap.add_argument("--cpu-threads", type=int, default=16, help="host threads split among the workers")
     ap.add_argument("--gen-seconds", type=float, default=1e9, help="data-generation deadline (seconds)")
     ap.add_argument("--epochs", type=int, default=40)
@@ -201,12 +225,16 @@ def main(argv=None):
     t0 = time.time()
     log(f"generating training data: {a.clusters} clusters, generator {a.generator}")
     gen_stats = {}
-    if a.generator == "batched":
+    if a.workers > 1:
+        X, T, M, done, gen_stats = _generate_parallel(a)
+    elif a.generator == "batched":
         from . import learned as lg
-        X, T, M = lg.make_training_data_batched(range(a.clusters), a.base_seed, log=log, stats=gen_stats)
+        from . import jk
+        X, T, M = lg.make_training_data_batched(range(a.clusters), a.base_seed, log=log, stats=gen_stats,
+                                                prec=jk.PREC_NAMES[a.gen_prec],
+                                                budget_bytes=int(a.budget_gib * (1 << 30)),
+                                                setup_threads=a.setup_threads, deadline=t0 + a.gen_seconds)
         done = gen_stats["clusters_done"]
-    elif a.workers > 1:
-        X, T, M, done = _generate_parallel(a)
     else:
         from . import learned as lg
         stats = {}
@@ -220,6 +248,7 @@ def main(argv=None):
     hist = lg.train(net, X, T, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
     meta = {"clusters": int(done), "clusters_requested": a.clusters, "base_seed": a.base_seed,
             "samples": int(len(X)), "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "generator": a.generator,
+            "gen_prec": a.gen_prec,
             "workers": a.workers, "generation_stats": gen_stats,
             "train_loss": hist["train"][-1], "val_loss": hist["val"][-1], "generation_seconds": t_gen,
             "train_seconds": time.time() - t0 - t_gen}
