@@ -316,7 +316,7 @@ struct BfHalf {
     static constexpr int LDS_ELEMS = S * (WGX + WGY) * TK;
     static constexpr int S_ = S, TK_ = TK;
     static_assert(LX >= 1 && LY >= 1 && LX * NT == WGX * NPC && LY * NT == WGY * NPC, "load shape");
-    __device__ static constexpr int sw(int r) { return NPC == 4 ? (((r >> 1) ^ (r >> 3)) & 3) : ((r ^ (r >> 3)) & 7); }
+    __device__ static constexpr int sw(int r) { return NPC == 4 ? ((r >> 1) & 3) : ((r >> 1) & 7); }
 };
 
 template <class H>
