@@ -17,7 +17,11 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;           // 4 waves, 2×2
 constexpr int TK = kTileK;            // 16
-constexpr int kPadA = 4;              // LDS row padding (elements)
+// LDS row padding (elements).  FP64: rows of 128 + 2 (or 64 + 2) doubles put the 16 rows that
+// one transposed stash instruction writes (one element each, 16 lanes at a time) on 16 distinct
+// 4-bank groups, and the 16 consecutive doubles of a fragment read on consecutive banks; a pad of
+// 4 leaves rows k and k + 8 on the same banks.  FP32 keeps 4.
+template <typename T> struct Pad { static constexpr int v = sizeof(T) == 8 ? 2 : 4; };
 constexpr int kXcd = 8;               // MI355X: 8 XCDs, work-groups dealt round-robin by linear id
 
 __device__ inline size_t tri(size_t j) { return j * (j + 1) / 2; }
@@ -64,8 +68,8 @@ __device__ __forceinline__ void khalf_tile(const T* __restrict__ bcol, int nbf, 
     using MM = Mma<T>;
     constexpr int TM = 2 * FM * 16, TN = 2 * FN * 16;
     constexpr int NA = TK * TM / kBlock, NB = TK * TN / kBlock;
-    __shared__ T As[2][TK][TM + kPadA];
-    __shared__ T Bs[2][TK][TN + kPadA];
+    __shared__ T As[2][TK][TM + Pad<T>::v];
+    __shared__ T Bs[2][TK][TN + Pad<T>::v];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     T ra[NA], rb[NB];
@@ -186,8 +190,8 @@ __device__ __forceinline__ void ksyrk_tile(const T* __restrict__ m, int ldm, int
     using MM = Mma<T>;
     constexpr int Tt = 2 * F * 16;
     constexpr int NL = TK * Tt / kBlock;
-    __shared__ T As[2][TK][Tt + kPadA];
-    __shared__ T Bs[2][TK][Tt + kPadA];
+    __shared__ T As[2][TK][Tt + Pad<T>::v];
+    __shared__ T Bs[2][TK][Tt + Pad<T>::v];
     __shared__ T Ss[2][TK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
