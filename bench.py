@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="scf", choices=["scf", "jk", "batched"])
+    ap.add_argument("--mode", default="scf", choices=["scf", "jk", "batched", "cpu"])
     ap.add_argument("--clusters", type=int, default=512, help="batched mode: number of water clusters")
     ap.add_argument("--batch-prec", default="bf16x3", choices=["fp64", "fp32", "bf16x1", "bf16x2", "bf16x3"],
                     help="batched mode: J/K precision of the batch")
@@ -83,38 +83,126 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(shape, seed, c, d, threads):
-    """Reference-structured CPU J/K (oracle) on a bounded column sample, scaled to naux."""
+def cpu_threads(requested: int) -> tuple:
+    """(threads used, nproc): every host core up to 16, the CPU share of one GPU on the GPU box."""
+    nproc = os.cpu_count() or 1
+    return (requested or min(nproc, 16)), nproc
+
+
+def cpu_medians(shape, seed, threads, reps=5):
+    """Reference-structured CPU J, occupied-route K and density-only K over every auxiliary column
+    of a synthetic B of `shape`: one warm-up, then the median of `reps` builds each (seconds)."""
     from oracle import oracle as orc
+    from rest_amd import workload as wl
+    orc.load_host_blas()
+    b = orc.synth_rimatr(seed, shape.nbf, shape.naux)
+    c = wl.random_orthonormal(seed, shape.nbf, shape.nocc)
+    occ = np.full(shape.nocc, 2.0)
+    d = (c * occ) @ c.T
+    out = {}
+    for name, fn in (("j", lambda: orc.cpu_baseline_vj(b, shape.nbf, d, threads)[0]),
+                     ("k_occ", lambda: orc.cpu_baseline_vk(b, shape.nbf, c, occ, threads)[0]),
+                     ("k_dm", lambda: orc.cpu_baseline_vk_dm(b, shape.nbf, d, threads)[0])):
+        fn()
+        out[name + "_s"] = float(np.median([fn() for _ in range(reps)]))
+    return out
+
+
+def cpu_baseline(shape, seed, c, d, threads):
+    """Reference-structured CPU J/K (oracle): the headline shape sampled over a bounded number of
+    auxiliary columns and scaled to naux (an estimate), plus full medians at the one-water and
+    (H2O)16 shapes."""
+    from oracle import oracle as orc
+    from rest_amd import workload as wl
     blas = orc.load_host_blas()
-    nthreads = threads or min(os.cpu_count() or 1, 16)
-    ncol_j, ncol_k, reps = 256, 128, 3
+    nthreads, nproc = cpu_threads(threads)
+    ncol_j, ncol_k, ncol_dm, reps = 256, 128, 32, 3
     bj = orc.synth_rimatr(seed, shape.nbf, shape.naux, 0, ncol_j)
-    tj = []
-    for _ in range(reps):
-        t, _ = orc.cpu_baseline_vj(bj, shape.nbf, d, nthreads)
-        tj.append(t)
+    tj = [orc.cpu_baseline_vj(bj, shape.nbf, d, nthreads)[0] for _ in range(reps)]
     del bj
     bk = orc.synth_rimatr(seed, shape.nbf, shape.naux, 0, ncol_k)
     occ = np.ones(c.shape[1])
-    tk = []
-    for _ in range(reps):
-        t, _ = orc.cpu_baseline_vk(bk, shape.nbf, c, occ, nthreads)
-        tk.append(t)
-    del bk
+    tk = [orc.cpu_baseline_vk(bk, shape.nbf, c, occ, nthreads)[0] for _ in range(reps)]
+    bd = bk[:, :ncol_dm]
+    tdm = [orc.cpu_baseline_vk_dm(bd, shape.nbf, d, nthreads)[0] for _ in range(reps)]
+    del bk, bd
     t_j = float(np.median(tj)) * shape.naux / ncol_j
     t_k = float(np.median(tk)) * shape.naux / ncol_k
+    t_dm = float(np.median(tdm)) * shape.naux / ncol_dm
+    full = {s_.name: cpu_medians(s_, 1, nthreads) for s_ in (wl.H2O, wl.H2O_16)}
+    # REST's RI-HF builds K through the density-only route when use_ri_symm is on (its default for
+    # RI-V, src/scf_io/mod.rs:2027-2029): the baseline build is J + density-only K
     return {
-        "value": 1.0 / (t_j + t_k),
+        "value": 1.0 / (t_j + t_dm),
         "unit": "J/K builds/s",
+        "route": "J + density-only K (vk_upper_with_rimatr_use_dm_only_sync); occupied-route K in k_seconds_scaled",
         "cores": nthreads,
+        "nproc": nproc,
         "kind": "port",
-        "sample": (f"oracle reference-structured J on {ncol_j} and K on {ncol_k} of {shape.naux} aux columns "
-                   f"(median of {reps}), scaled linearly to all columns; per-P BLAS: "
-                   f"{blas or 'built-in loops'} single-threaded inside {nthreads} OpenMP tasks"),
+        "sample": (f"oracle reference-structured J on {ncol_j}, occupied-route K on {ncol_k} and density-only K on "
+                   f"{ncol_dm} of {shape.naux} aux columns (median of {reps}), scaled linearly to all columns "
+                   f"(an estimate); per-P BLAS: {blas or 'built-in loops'} single-threaded inside {nthreads} OpenMP "
+                   f"tasks ({nproc} cores reported by the OS)"),
         "j_seconds_scaled": t_j,
         "k_seconds_scaled": t_k,
+        "k_dm_seconds_scaled": t_dm,
+        "full_medians_s": full,
     }
+
+
+def main_cpu(args):
+    """CPU baseline protocol: reference-structured J, occupied-route K and density-only K (median of
+    5 after one warm-up, every auxiliary column) at the one-water and (H2O)16 shapes, the (H2O)64
+    shape as a scaled sample, and the whole RI-HF SCF on the CPU (REST's loop restated in
+    oracle/scf_oracle.py, J and density-only K from the same reference-structured routines, FP64) for
+    the single water and (H2O)16 seed 1 from the SAD density."""
+    from oracle import oracle as orc
+    from oracle import scf_oracle as so
+    from rest_amd import guess as gs
+    from rest_amd import ri
+    from rest_amd import workload as wl
+    blas = orc.load_host_blas()
+    nthreads, nproc = cpu_threads(args.cpu_threads)
+    out = {"mode": "cpu", "threads": nthreads, "nproc": nproc, "blas": blas or "built-in loops",
+           "builds": {}, "scf": {}}
+    for shape in (wl.H2O, wl.H2O_16):
+        out["builds"][shape.name] = cpu_medians(shape, 1, nthreads)
+        print(f"[bench] cpu builds {shape.name}: {out['builds'][shape.name]}", file=sys.stderr, flush=True)
+    c3 = wl.H2O_64
+    c = wl.random_orthonormal(2, c3.nbf, c3.nocc)
+    cb = cpu_baseline(c3, 2, c, c @ c.T, nthreads)
+    out["builds"][c3.name + " (scaled estimate)"] = {"j_s": cb["j_seconds_scaled"], "k_occ_s": cb["k_seconds_scaled"],
+                                                     "k_dm_s": cb["k_dm_seconds_scaled"], "sample": cb["sample"]}
+    mono = ri.Molecule.from_angstrom(["O", "H", "H"], np.array([[0.0, 0.0, 0.0], [0.7572, 0.5865, 0.0],
+                                                                [-0.7572, 0.5865, 0.0]]))
+    for name, mol in (("H2O", mono), ("(H2O)16 seed 1", ri.Molecule.water_cluster(wl.water_cluster(1, 16)))):
+        t0 = time.perf_counter()
+        S, T, V = mol.int1e()
+        B, kept = so.rimatr_from_integrals(mol.int3c2e_packed(), mol.int2c2e(), 1e-8)
+        t_setup = time.perf_counter() - t0
+        da, db = gs.sad_density(mol)
+        jk_s = [0.0]
+
+        def jk(D, B=B, n=mol.nbf):
+            a = time.perf_counter()
+            _, vj = orc.cpu_baseline_vj(B, n, D, nthreads)
+            _, vk = orc.cpu_baseline_vk_dm(B, n, D, nthreads)
+            jk_s[0] += time.perf_counter() - a
+            iu = np.triu_indices(n)
+            order = np.lexsort((iu[0], iu[1]))
+            J, K = np.zeros((n, n)), np.zeros((n, n))
+            J[iu[0][order], iu[1][order]] = vj
+            K[iu[0][order], iu[1][order]] = vk
+            return J + np.triu(J, 1).T, K + np.triu(K, 1).T
+
+        t1 = time.perf_counter()
+        r = so.rhf(S, T + V, B, mol.nelectron, mol.nuclear_repulsion(), so.ScfCtrl(), d_guess=da + db, jk=jk)
+        t_scf = time.perf_counter() - t1
+        out["scf"][name] = {"nbf": mol.nbf, "naux": mol.naux, "energy": r.energy, "converged": bool(r.converged),
+                            "iterations": r.iterations, "scf_s": t_scf, "jk_s": jk_s[0],
+                            "setup_s_integrals_and_b": t_setup}
+        print(f"[bench] cpu scf {name}: {out['scf'][name]}", file=sys.stderr, flush=True)
+    print(json.dumps(out))
 
 
 def load_traffic(kernel: str):
@@ -138,6 +226,8 @@ def main():
         return main_scf(args)
     if args.mode == "batched":
         return main_batched(args)
+    if args.mode == "cpu":
+        return main_cpu(args)
     return main_jk(args)
 
 
@@ -292,9 +382,11 @@ def main_scf(args):
         shape = wl.H2O_64 if n == 64 else wl.H2O_16
         c_occ = wl.random_orthonormal(seed, shape.nbf, shape.nocc)
         cb = cpu_baseline(shape, seed, c_occ, c_occ @ c_occ.T, args.cpu_threads)
-        per_build = cb["j_seconds_scaled"] + cb["k_seconds_scaled"]
+        per_build = cb["j_seconds_scaled"] + cb["k_dm_seconds_scaled"]
         cpu = {"value": per_build * (op.iterations + 1), "unit": "s (plain FP64 SCF, J/K only)",
-               "cores": cb["cores"], "kind": cb["kind"],
+               "cores": cb["cores"], "nproc": cb["nproc"], "kind": cb["kind"], "route": cb["route"],
+               "j_seconds_scaled": cb["j_seconds_scaled"], "k_dm_seconds_scaled": cb["k_dm_seconds_scaled"],
+               "k_occ_seconds_scaled": cb["k_seconds_scaled"], "full_medians_s": cb["full_medians_s"],
                "sample": cb["sample"] + f"; times the {op.iterations + 1} J/K builds of the plain FP64 SCF "
                                         "(diagonalisation and DIIS not included)",
                "gpu_plain_scf_s": t_plain}

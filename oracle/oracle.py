@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
             "orc_blas_load": (i, [C.c_char_p, C.c_char_p, C.c_char_p]),
             "orc_cpu_baseline_vj": (d, [vp, sz, sz, vp, vp, i]),
             "orc_cpu_baseline_vk": (d, [vp, sz, sz, vp, sz, vp, vp, i]),
+            "orc_cpu_baseline_vk_dm": (d, [vp, sz, sz, vp, vp, i]),
             "orc_max_threads": (i, []),
         }
         for name, (res, args) in sig.items():
@@ -215,6 +216,16 @@ def cpu_baseline_vk(rimatr_cols: np.ndarray, nbf: int, c: np.ndarray, occ: np.nd
     o = np.ascontiguousarray(occ, dtype=np.float64)
     out = np.empty(nbf * (nbf + 1) // 2)
     t = lib().orc_cpu_baseline_vk(_p(b), nbf, b.shape[0], _p(cc), nw, _p(o), _p(out), threads)
+    return t, out
+
+
+def cpu_baseline_vk_dm(rimatr_cols: np.ndarray, nbf: int, dm: np.ndarray, threads: int = 0):
+    """Reference-structured density-only K (REST's use_dm_only route) over the given packed columns;
+    returns (seconds, vk)."""
+    b = _rimatr_buf(rimatr_cols)
+    d = _cm([dm], 1, (nbf, nbf))
+    out = np.empty(nbf * (nbf + 1) // 2)
+    t = lib().orc_cpu_baseline_vk_dm(_p(b), nbf, b.shape[0], _p(d), _p(out), threads)
     return t, out
 
 

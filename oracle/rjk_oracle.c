@@ -553,6 +553,44 @@ double orc_cpu_baseline_vk(const double* b, size_t nbf, size_t ncol, const doubl
 #endif
 }
 
+/* K as REST's density-only route runs it (a4, vk_upper_with_rimatr_use_dm_only_sync,
+ * src/scf_io/mod.rs:2703-2756): per P unpack, T = Bsym·D and K_P = T·Bsym (REST's two
+ * dsymm calls, here as dgemm on the unpacked symmetric B_P), upper filter, serial
+ * accumulation. */
+double orc_cpu_baseline_vk_dm(const double* b, size_t nbf, size_t ncol, const double* dm, double* vk,
+                              int threads) {
+    const size_t npair = nbf * (nbf + 1) / 2;
+#ifdef _OPENMP
+    double t0 = omp_get_wtime();
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    if (g_setthreads) g_setthreads(1);
+    memset(vk, 0, sizeof(double) * npair);
+#pragma omp parallel
+    {
+        double* bp = (double*)malloc(sizeof(double) * nbf * nbf);
+        double* T = (double*)malloc(sizeof(double) * nbf * nbf);
+        double* kf = (double*)malloc(sizeof(double) * nbf * nbf);
+        double* up = (double*)malloc(sizeof(double) * npair);
+#pragma omp for schedule(dynamic, 1)
+        for (long long P = 0; P < (long long)ncol; ++P) {
+            unpack_sym(b + (size_t)P * npair, nbf, bp);
+            bl_gemm('N', 'N', nbf, nbf, nbf, bp, nbf, dm, nbf, 0.0, T, nbf);
+            bl_gemm('N', 'N', nbf, nbf, nbf, T, nbf, bp, nbf, 0.0, kf, nbf);
+            for (size_t j = 0; j < nbf; ++j)
+                for (size_t i = 0; i <= j; ++i) up[pidx(i, j)] = kf[i + j * nbf];
+#pragma omp critical
+            for (size_t p = 0; p < npair; ++p) vk[p] += up[p];
+        }
+        free(bp); free(T); free(kf); free(up);
+    }
+#ifdef _OPENMP
+    return omp_get_wtime() - t0;
+#else
+    return 0;
+#endif
+}
+
 int orc_max_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

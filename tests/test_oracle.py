@@ -336,3 +336,22 @@ def test_fock_is_energy_gradient(uhf):
         fd = (_energy_functional(bfull, h, dp, uhf) - _energy_functional(bfull, h, dmn, uhf)) / (2 * eps)
         analytic = np.sum(unpack(f[s], nbf) * dd)
         np.testing.assert_allclose(analytic, fd, rtol=1e-7, atol=1e-7)
+
+
+def test_cpu_baseline_routes_equal_the_oracle():
+    """The reference-structured CPU J/K that bench.py times (per-P BLAS calls, serial accumulation)
+    compute the same J, occupied-route K and density-only K as the compensated oracle."""
+    nbf, naux, nocc = 37, 23, 6
+    b = orc.synth_rimatr(5, nbf, naux)
+    c = orc.random_orthonormal(6, nbf, nocc)
+    occ = np.full(nocc, 2.0)
+    d = (c * occ) @ c.T
+    orc.load_host_blas()
+    _, vj = orc.cpu_baseline_vj(b, nbf, d, 2)
+    _, vk = orc.cpu_baseline_vk(b, nbf, c, occ, 2)
+    _, vkd = orc.cpu_baseline_vk_dm(b, nbf, d, 2)
+    rj = orc.vj_upper_rimatr(b, [d], 1, 1.0)[0]
+    rk = orc.vk_upper_occ(b, True, [c], [nocc - 1], [occ], 1, 1.0)[0]
+    rkd = orc.vk_upper_dm(b, True, [d], 1, 1.0)[0]
+    for got, ref in ((vj, rj), (vk, rk), (vkd, rkd), (vkd, rk)):
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
