@@ -127,27 +127,29 @@ __global__ void k_bf_prep_c_idx(const double* __restrict__ vecs, int nbf, const 
 }
 
 // Tile shapes from a sweep at nbf 1536, nk 320, 256 auxiliary columns per launch (executed
-// BF16 MFMA rate, accumulators in VGPRs):
-//   half-transform, 2 planes: 64 c × 256 μ, one wave per 64 × 64, 64-deep: 902 TFLOP/s
-//     (64 × 128 on 2 × 2 waves, 64-deep: 734; 32-deep double-buffered: 737); needs
-//     nbf_pad % 256 == 0, else the 64 × 128 tile;
-//   half-transform, 3 planes: 64 c × 128 μ on 2 × 2 waves, 64-deep: 965 (64 × 256: 848);
-//   SYRK, 2 planes: 128 × 128 on 2 × 2 waves, 64-deep: 968 (32-deep: 944);
-//   SYRK, 3 planes: 128 × 128 on 2 × 4 waves, 32-deep: 1094 (2 × 2 waves: 1074).
+// BF16 MFMA rate, accumulators in VGPRs, the conflict-free LDS swizzle of rjk_bftile.hpp), then
+// checked in the K build itself (about 1000 auxiliary columns per launch):
+//   half-transform, 2 planes: 64 c × 256 μ on 1 × 4 waves, 64-deep: 5.39 ms per launch in the
+//     build against 5.46 for 64 × 128 on 2 × 2 waves (the sweep, at a quarter of the columns,
+//     ranks them the other way: 747 against 857 TFLOP/s); needs nbf_pad % 256 == 0, else 64 × 128;
+//   half-transform, 3 planes: 64 c × 128 μ on 2 × 4 waves, 64-deep: 9.34 ms per 1024 columns
+//     (2 × 2 waves: 9.64; 64 × 256 on 2 × 4: 10.24);
+//   SYRK, 2 planes: 128 × 128 on 2 × 4 waves, 64-deep: 988 (2 × 2 waves: 951; in the build 2.66
+//     against 2.93 ms per launch);
+//   SYRK, 3 planes: 128 × 128 on 2 × 4 waves, 32-deep: 1110 (2 × 2 waves: 1077; 64-deep: 1043).
 template <int S> struct KbfTiles;
 template <> struct KbfTiles<2> {
     using Half = bft::Cfg<2, 64, 256, 1, 4, false, 64>;    // X = 64 c, Y = 256 μ
     using HalfNarrow = bft::Cfg<2, 64, 128, 2, 2, false, 64>;
-    using Syrk = bft::Cfg<2, 128, 128, 2, 2, false, 64>;   // X = 128 ν, Y = 128 μ
+    using Syrk = bft::Cfg<2, 128, 128, 2, 4, false, 64>;   // X = 128 ν, Y = 128 μ
 };
 template <> struct KbfTiles<1> {
     using Half = bft::Cfg<1, 64, 256, 1, 4, false, 64>;
     using HalfNarrow = bft::Cfg<1, 64, 128, 2, 2, false, 64>;
-    using Syrk = bft::Cfg<1, 128, 128, 2, 2, false, 64>;
+    using Syrk = bft::Cfg<1, 128, 128, 2, 4, false, 64>;
 };
 template <> struct KbfTiles<3> {
-    using Half = bft::Cfg<3, 64, 128, 2, 2, false, 64>;
-    using HalfNarrow = Half;
+    using Half = bft::Cfg<3, 64, 128, 2, 4, false, 64>;
     using Syrk = bft::Cfg<3, 128, 128, 2, 4, false, 32>;
 };
 
