@@ -70,8 +70,9 @@ size_t rjk_batch_bytes_b(const rjk_batch* batch);
 int rjk_batch_set_stream(rjk_batch* batch, void* stream);
 /* B of problem i from the host, REST's rimatr layout [naux][npair]. */
 int rjk_batch_set_rimatr(rjk_batch* batch, size_t i, const double* rimatr);
-/* B of problem i from a device array in the padded rimatr layout [naux][ld], ld >= npair (for
- * example rjk_rimatr_dev of an rjk_ctx with ld = rjk_ld_baspar); converted to the batch precision. */
+/* B of problem i from a device array in the padded rimatr layout [naux][ld], ld >= npair, holding
+ * at least naux columns of problem i (for example rjk_rimatr_dev of an rjk_ctx of the same nbf and
+ * naux, with ld = rjk_ld_baspar); converted to the batch precision.  Synchronises. */
 int rjk_batch_set_rimatr_dev(rjk_batch* batch, size_t i, const double* rimatr_dev, size_t ld);
 /* Every problem's B from the seeded synthetic generator of rjk_fill_rimatr_synthetic (seed + i). */
 int rjk_batch_fill_synthetic(rjk_batch* batch, uint64_t seed);

@@ -24,6 +24,7 @@ Molecules go through in waves whose device memory fits a budget
 """
 from __future__ import annotations
 
+import copy
 import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -179,86 +180,89 @@ def rhf_batch(mols, ctrl=None, prec: int = jk.PREC_FP64, budget_bytes: int = 0, 
     def tadd(key, v):
         with tlock:
             st[key] += v
-    pool = ThreadPoolExecutor(max_workers=max(1, threads))
-    store = jk.STORE_FP64
-    try:
+
+    def run_wave(batch, i0, i1, pool):
+        t0 = time.perf_counter()
+        loops, pend = {}, {}
+
+        def setup(k):
+            i = i0 + k
+            m = mols[i]
+            ta = time.perf_counter()
+            S, T, V = m.int1e()
+            H = T + V
+            tb = time.perf_counter()
+            with jk.DeviceRI(m.nbf, m.naux) as r:
+                m.build_rimatr(r, 1e-8, jk.STORE_FP64)
+                tc = time.perf_counter()
+                c1 = gs.sad_guess(m, r, S, H).c1
+                batch.set_from_ri(k, r)
+            td = time.perf_counter()
+            tadd("setup_int1e_thread_s", tb - ta)
+            tadd("setup_b_thread_s", tc - tb)
+            tadd("setup_guess_thread_s", td - tc)
+            c1 = c1[0] if c1.ndim == 3 else c1
+            loop = _rhf_loop(S, H, m.nuclear_repulsion(), nocc[i], c1, ctrl)
+            return k, loop, next(loop)
+
+        with ThreadPoolExecutor(max_workers=max(1, setup_threads)) as ex, \
+                threadpool_limits(limits=max(1, threads // max(1, setup_threads))):
+            for k, loop, req in ex.map(setup, range(i1 - i0)):
+                loops[k] = loop
+                pend[k] = req
+        st["setup_seconds"] += time.perf_counter() - t0
+        occs = {k: np.full(nocc[i0 + k], 2.0) for k in loops}
+
+        def resume(k, jkm):
+            try:
+                return k, loops[k].send(jkm), None
+            except StopIteration as e:
+                return k, None, e.value
+            except _NeedsMixing:
+                return k, None, "mixing"
+
+        while pend:
+            t1 = time.perf_counter()
+            for k, co in pend.items():
+                batch.set_inputs(k, 2.0 * co @ co.T, co, occs[k])
+            batch.run(1.0, 1.0)
+            batch.synchronize()
+            res = {k: batch.get_outputs(k) for k in pend}
+            t2 = time.perf_counter()
+            st["batch_runs"] += 1
+            st["jk_seconds"] += t2 - t1
+            with threadpool_limits(limits=1):
+                done = list(pool.map(lambda k: resume(k, (_unpack(res[k][0], nbf[i0 + k]),
+                                                          _unpack(res[k][1], nbf[i0 + k]))), list(pend)))
+            st["host_seconds"] += time.perf_counter() - t2
+            new = {}
+            for k, req, fin in done:
+                if req is not None:
+                    new[k] = req
+                    continue
+                batch.set_active(k, False)
+                i = i0 + k
+                if fin == "mixing":
+                    st["fallbacks"] += 1
+                    out[i] = _single(mols[i], ctrl)
+                else:
+                    E, conv, it, D, C, eps, energies = fin
+                    out[i] = BatchOutcome(E, bool(conv), it, D, C, eps, True, energies)
+            pend = new
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as pool:
         for w, (i0, i1) in enumerate(waves):
             t0 = time.perf_counter()
             batch = jk.Batch(nbf[i0:i1], naux[i0:i1], nocc[i0:i1], prec)
             st["alloc_seconds"] += time.perf_counter() - t0
-            loops, pend = {}, {}
-
-            def setup(k):
-                i = i0 + k
-                m = mols[i]
-                ta = time.perf_counter()
-                S, T, V = m.int1e()
-                H = T + V
-                tb = time.perf_counter()
-                with jk.DeviceRI(m.nbf, m.naux) as r:
-                    m.build_rimatr(r, 1e-8, store)
-                    tc = time.perf_counter()
-                    c1 = gs.sad_guess(m, r, S, H).c1
-                    batch.set_from_ri(k, r)
-                td = time.perf_counter()
-                tadd("setup_int1e_thread_s", tb - ta)
-                tadd("setup_b_thread_s", tc - tb)
-                tadd("setup_guess_thread_s", td - tc)
-                c1 = c1[0] if c1.ndim == 3 else c1
-                loop = _rhf_loop(S, H, m.nuclear_repulsion(), nocc[i], c1, ctrl)
-                return k, loop, next(loop)
-
-            with ThreadPoolExecutor(max_workers=max(1, setup_threads)) as ex, \
-                    threadpool_limits(limits=max(1, threads // max(1, setup_threads))):
-                for k, loop, req in ex.map(setup, range(i1 - i0)):
-                    loops[k] = loop
-                    pend[k] = req
-            st["setup_seconds"] += time.perf_counter() - t0
-            occs = {k: np.full(nocc[i0 + k], 2.0) for k in loops}
-
-            def resume(k, jkm):
-                try:
-                    return k, loops[k].send(jkm), None
-                except StopIteration as e:
-                    return k, None, e.value
-                except _NeedsMixing:
-                    return k, None, "mixing"
-
-            while pend:
-                t1 = time.perf_counter()
-                for k, co in pend.items():
-                    batch.set_inputs(k, 2.0 * co @ co.T, co, occs[k])
-                batch.run(1.0, 1.0)
-                batch.synchronize()
-                res = {k: batch.get_outputs(k) for k in pend}
-                t2 = time.perf_counter()
-                st["batch_runs"] += 1
-                st["jk_seconds"] += t2 - t1
-                with threadpool_limits(limits=1):
-                    done = list(pool.map(lambda k: resume(k, (_unpack(res[k][0], nbf[i0 + k]),
-                                                              _unpack(res[k][1], nbf[i0 + k]))), list(pend)))
-                st["host_seconds"] += time.perf_counter() - t2
-                new = {}
-                for k, req, fin in done:
-                    if req is not None:
-                        new[k] = req
-                        continue
-                    batch.set_active(k, False)
-                    i = i0 + k
-                    if fin == "mixing":
-                        st["fallbacks"] += 1
-                        out[i] = _single(mols[i], ctrl)
-                    else:
-                        E, conv, it, D, C, eps, energies = fin
-                        out[i] = BatchOutcome(E, bool(conv), it, D, C, eps, True, energies)
-                pend = new
-            t3 = time.perf_counter()
-            batch.close()
-            st["alloc_seconds"] += time.perf_counter() - t3
+            try:
+                run_wave(batch, i0, i1, pool)
+            finally:
+                t3 = time.perf_counter()
+                batch.close()
+                st["alloc_seconds"] += time.perf_counter() - t3
             if log:
                 log(f"wave {w + 1}/{len(waves)}: molecules {i0}..{i1 - 1} done, {st['batch_runs']} batch runs so far")
-    finally:
-        pool.shutdown()
     if stats is not None:
         stats.update(st)
     return out
@@ -266,5 +270,5 @@ def rhf_batch(mols, ctrl=None, prec: int = jk.PREC_FP64, budget_bytes: int = 0, 
 
 def _single(mol, ctrl) -> BatchOutcome:
     """The single-problem device SCF of one molecule (the fallback for a mixing step)."""
-    o, _ = rs.run_molecule(mol, guess="sad", ctrl=ctrl)
+    o, _ = rs.run_molecule(mol, guess="sad", ctrl=copy.copy(ctrl))
     return BatchOutcome(o.energy, bool(o.converged), o.iterations, o.dm, o.mo_coeff, o.mo_energy, False, [])

@@ -448,7 +448,10 @@ class Batch:
         self._check(self.lib.rjk_batch_set_rimatr_dev(self.h, i, C.c_void_p(ptr), int(ld)))
 
     def set_from_ri(self, i: int, ri: "DeviceRI"):
-        """B of problem i copied on the device from a DeviceRI's FP64 storage."""
+        """B of problem i copied on the device from a DeviceRI's FP64 storage (same nbf and naux)."""
+        if ri.nbf != self.nbf[i] or ri.naux != self.naux[i]:
+            raise ValueError(f"set_from_ri: DeviceRI is {ri.nbf} x {ri.naux}, problem {i} is "
+                             f"{self.nbf[i]} x {self.naux[i]}")
         self.set_rimatr_dev(i, ri.rimatr_dev_ptr(), ri.ldb)
 
     def set_active(self, i: int, active: bool):

@@ -45,12 +45,21 @@ def test_pentacene_device_scf_matches_rest_and_oracle():
     assert out.converged
     assert abs(out.energy - (-831.0241601906)) < 1e-8
     say("pentacene tight thresholds")
-    out2, _ = rs.run_molecule(pentacene(), ctrl=tight())
+    mol = pentacene()
+    out2, t2 = rs.run_molecule(mol, ctrl=tight(), keep_b=True)
     ref = E_GOLD["pentacene_sto3g_def2svpjkfit"]["energy"]
     say(f"  E {out2.energy:.12f} vs oracle {ref:.12f} (oracle B from numpy's V^-1/2)")
     # the golden run builds V^{-1/2} with numpy, the device with rocSOLVER; the metric's
     # condition number (~1e7) turns their 1e-16-level differences into ~1e-9 Eh
     assert out2.converged and abs(out2.energy - ref) <= 5e-9
+    # on the identical B (read back from HBM), the oracle loop from the device's converged density
+    from oracle import scf_oracle as so
+    from tests.test_integrals import numpy_jk
+    octrl = so.ScfCtrl(scf_acc_etot=1e-11, scf_acc_rho=1e-9, scf_acc_eev=1e-8, max_scf_cycle=200)
+    same = so.rhf(t2["S"], t2["H"], t2["rimatr"], mol.nelectron, mol.nuclear_repulsion(), octrl, d_guess=out2.dm,
+                  jk=numpy_jk(t2["rimatr"], mol.nbf))
+    say(f"  identical B: device {out2.energy:.12f} oracle {same.energy:.12f} diff {out2.energy - same.energy:.2e}")
+    assert same.converged and abs(out2.energy - same.energy) <= 1e-9
 
 
 def test_pentacene_mixed_ladder_matches_rest():
