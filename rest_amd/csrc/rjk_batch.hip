@@ -303,14 +303,14 @@ __global__ __launch_bounds__(256) void k_b_bf_prep(const BProb* __restrict__ pr,
     }
 }
 
-// Half-transform tile D[c][μ] = Σ_ν C'ᵀ[c][ν] B[μ][ν] of 64 c × TM μ on 2 × 2 waves, k-blocks of TK
+// Half-transform tile D[c][μ] = Σ_ν C'ᵀ[c][ν] B[μ][ν] of TN c × TM μ on 2 × 2 waves, k-blocks of TK
 // ν.  B comes from the packed FP32 column (element (i<=j) at j(j+1)/2 + i); each thread stages
 // runs of 8 consecutive ν of one μ: along ν (contiguous in memory) when the tile lies wholly below
 // the diagonal, else one μ per lane (contiguous across lanes above the diagonal).  LDS rows are TK
 // BF16 values, 16-byte piece p of row r at p ^ sw(r) (the swizzle of rjk_bftile.hpp).
-template <int S, int TM, int TK>
+template <int S, int TM, int TK, int TN = 64>
 struct BfHalf {
-    static constexpr int NT = 256, WX = 2, WY = 2, WGX = 64, WGY = TM, NPC = TK / 8;
+    static constexpr int NT = 256, WX = 2, WY = 2, WGX = TN, WGY = TM, NPC = TK / 8;
     static constexpr int FX = WGX / WX / 16, FY = WGY / WY / 16;
     static constexpr int LX = WGX * NPC / NT, LY = WGY * NPC / NT;
     static constexpr int LDS_ELEMS = S * (WGX + WGY) * TK;
@@ -601,6 +601,13 @@ size_t prob_bytes(const BProb& P, int prec) {
 
 bool valid_prec(int prec) { return prec == RJK_PREC_FP64 || prec == RJK_PREC_FP32 || is_bf(prec); }
 
+// Half-transform groups: (μ rows, c columns) of the tile.  BF16 uses all five (one c tile covers
+// every occupied column up to 192 at 64 rows, 128 at 128 rows, so each B element is staged once);
+// FP64/FP32 use 0 and 3 (rjk_ktiles.hpp's 64 × 64 and 128 × 64 tiles).
+constexpr int kHalfGroups = 5;
+constexpr int kGroupTM[kHalfGroups] = {64, 64, 64, 128, 128};
+constexpr int kGroupTN[kHalfGroups] = {64, 128, 192, 64, 128};
+
 bool valid_shape(size_t nbf, size_t naux, size_t nocc) {
     return nbf != 0 && naux != 0 && nocc != 0 && nocc <= nbf && nbf <= 32768 && naux <= (1u << 24);
 }
@@ -616,9 +623,10 @@ struct rjk_batch {
     std::vector<BProb> h;            // host copy of the problem table
     BProb* d_prob = nullptr;
     size_t* d_tab = nullptr;         // the five prefix tables back to back
-    int* d_sub = nullptr;            // [2][nprob] problem lists of the two tile classes
-    size_t* d_suboff = nullptr;      // [2][nprob + 1] half-transform item offsets per class
-    size_t* d_syrkoff = nullptr;     // [2][nprob + 1] SYRK item offsets per class (BF16 precisions)
+    int* d_sub = nullptr;            // [kHalfGroups][nprob] problem lists of the half-transform groups
+    size_t* d_suboff = nullptr;      // [kHalfGroups][nprob + 1] their item offsets
+    int* d_ssub = nullptr;           // [2][nprob] problem lists of the two SYRK tile classes (BF16)
+    size_t* d_syrkoff = nullptr;     // [2][nprob + 1] their item offsets
     std::vector<size_t> t_pairs, t_cols, t_jseg, t_prep, t_syrk;
     std::vector<void*> allocs;
     double* scratch = nullptr;       // FP64 staging for uploads / the synthetic fill
@@ -628,8 +636,9 @@ struct rjk_batch {
     int khalf_cut = 256;             // nbf_pad below this uses the narrow tile class
     int waves = 2;                   // persistent work-groups per CU for the tile kernels
     int ncu = 256;
-    int t_sub_n[2] = {0, 0};
-    size_t t_sub_items[2] = {0, 0};
+    int t_sub_n[kHalfGroups] = {};
+    size_t t_sub_items[kHalfGroups] = {};
+    int t_ssub_n[2] = {0, 0};
     size_t t_syrk_items[2] = {0, 0};
     double sj = 1.0, sk = 1.0;
     hipGraph_t graph = nullptr;
@@ -674,41 +683,51 @@ int tile_class(const rjk_batch* b, const BProb& P) {
     return (is_bf(b->prec) && P.nbf_pad % 128) ? 0 : 1;
 }
 
+int half_group(const rjk_batch* b, const BProb& P) {
+    const int v = tile_class(b, P);
+    if (!is_bf(b->prec)) return v == 0 ? 0 : 3;
+    if (v == 0) return P.nk == 128 ? 1 : (P.nk == 192 ? 2 : 0);
+    return P.nk == 128 ? 4 : 3;
+}
+
 int build_subsets(rjk_batch* b) {
     const int np = (int)b->h.size();
     const bool bf = is_bf(b->prec);
-    std::vector<int> sub[2];
-    std::vector<size_t> off[2], soff[2];
-    for (int v = 0; v < 2; ++v) {
-        off[v].push_back(0);
-        soff[v].push_back(0);
-    }
+    std::vector<int> sub[kHalfGroups], ssub[2];
+    std::vector<size_t> off[kHalfGroups], soff[2];
+    for (int g = 0; g < kHalfGroups; ++g) off[g].push_back(0);
+    for (int v = 0; v < 2; ++v) soff[v].push_back(0);
     for (int q = 0; q < np; ++q) {
         const BProb& P = b->h[q];
-        const int v = tile_class(b, P);
-        const int TM = v == 0 ? 64 : 128, TN = 64;
-        sub[v].push_back(q);
-        off[v].push_back(off[v].back() + rup(P.naux, 8) * (size_t)(P.nbf_pad / TM) * (P.nk / TN));
-        const size_t nt = (size_t)P.nbf_pad / TM;
-        soff[v].push_back(soff[v].back() + (bf ? (size_t)P.nsplit * (nt * (nt + 1) / 2) : 0));
+        const int g = half_group(b, P), TM = kGroupTM[g], TN = kGroupTN[g];
+        sub[g].push_back(q);
+        off[g].push_back(off[g].back() + rup(P.naux, 8) * (size_t)(P.nbf_pad / TM) * (P.nk / TN));
+        if (bf) {
+            const int v = tile_class(b, P);
+            const size_t nt = (size_t)P.nbf_pad / (v == 0 ? 64 : 128);
+            ssub[v].push_back(q);
+            soff[v].push_back(soff[v].back() + (size_t)P.nsplit * (nt * (nt + 1) / 2));
+        }
     }
-    std::vector<int> hs(2 * np, 0);
-    std::vector<size_t> ho(2 * (np + 1), 0), hso(2 * (np + 1), 0);
+    // entries past a list's end stay zero (never searched: its length bounds the search)
+    std::vector<int> hs(kHalfGroups * np, 0), hss(2 * np, 0);
+    std::vector<size_t> ho(kHalfGroups * (np + 1), 0), hso(2 * (np + 1), 0);
+    for (int g = 0; g < kHalfGroups; ++g) {
+        std::copy(sub[g].begin(), sub[g].end(), hs.begin() + g * np);
+        std::copy(off[g].begin(), off[g].end(), ho.begin() + g * (np + 1));
+        b->t_sub_n[g] = (int)sub[g].size();
+        b->t_sub_items[g] = off[g].back();
+    }
     for (int v = 0; v < 2; ++v) {
-        std::copy(sub[v].begin(), sub[v].end(), hs.begin() + v * np);
-        std::copy(off[v].begin(), off[v].end(), ho.begin() + v * (np + 1));
+        std::copy(ssub[v].begin(), ssub[v].end(), hss.begin() + v * np);
         std::copy(soff[v].begin(), soff[v].end(), hso.begin() + v * (np + 1));
-        // entries past the subset stay zero (never searched: nsub bounds the search)
-    }
-    b->t_sub_n[0] = (int)sub[0].size();
-    b->t_sub_n[1] = (int)sub[1].size();
-    BHIP(b, hipMemcpyAsync(b->d_sub, hs.data(), hs.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    BHIP(b, hipMemcpyAsync(b->d_suboff, ho.data(), ho.size() * sizeof(size_t), hipMemcpyHostToDevice, b->stream));
-    BHIP(b, hipMemcpyAsync(b->d_syrkoff, hso.data(), hso.size() * sizeof(size_t), hipMemcpyHostToDevice, b->stream));
-    for (int v = 0; v < 2; ++v) {
-        b->t_sub_items[v] = off[v].back();
+        b->t_ssub_n[v] = (int)ssub[v].size();
         b->t_syrk_items[v] = soff[v].back();
     }
+    BHIP(b, hipMemcpyAsync(b->d_sub, hs.data(), hs.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_suboff, ho.data(), ho.size() * sizeof(size_t), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_ssub, hss.data(), hss.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_syrkoff, hso.data(), hso.size() * sizeof(size_t), hipMemcpyHostToDevice, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));   // host vectors die here
     return RJK_OK;
 }
@@ -744,9 +763,9 @@ int enqueue_fp(rjk_batch* b, hipEvent_t* ev) {
         hipLaunchKernelGGL((k_b_khalf<T, 2, 2>), dim3(grid_items(b, b->t_sub_items[0], b->waves)), dim3(kBlock), 0,
                            st, b->d_prob, b->d_sub, b->d_suboff, b->t_sub_n[0]);
     mark(5);
-    if (b->t_sub_items[1])
-        hipLaunchKernelGGL((k_b_khalf<T, 4, 2>), dim3(grid_items(b, b->t_sub_items[1], b->waves)), dim3(kBlock), 0,
-                           st, b->d_prob, b->d_sub + np, b->d_suboff + (np + 1), b->t_sub_n[1]);
+    if (b->t_sub_items[3])
+        hipLaunchKernelGGL((k_b_khalf<T, 4, 2>), dim3(grid_items(b, b->t_sub_items[3], b->waves)), dim3(kBlock), 0,
+                           st, b->d_prob, b->d_sub + 3 * np, b->d_suboff + 3 * (np + 1), b->t_sub_n[3]);
     mark(6);
     hipLaunchKernelGGL((k_b_ksyrk<T, 4>), dim3(grid_items(b, b->t_syrk.back(), b->waves)), dim3(kBlock), 0, st,
                        b->d_prob, np, tb);
@@ -761,8 +780,11 @@ int enqueue_fp(rjk_batch* b, hipEvent_t* ev) {
 template <int S>
 int enqueue_bf(rjk_batch* b, hipEvent_t* ev) {
     constexpr int TK = S == 3 ? 32 : 64;
-    using H0 = BfHalf<S, 64, TK>;
-    using H1 = BfHalf<S, 128, TK>;
+    using H0 = BfHalf<S, 64, TK, 64>;
+    using H1 = BfHalf<S, 64, TK, 128>;
+    using H2 = BfHalf<S, 64, TK, 192>;
+    using H3 = BfHalf<S, 128, TK, 64>;
+    using H4 = BfHalf<S, 128, TK, 128>;
     using Y0 = bft::Cfg<S, 64, 64, 2, 2, false, TK>;
     using Y1 = bft::Cfg<S, 128, 128, 2, 4, false, TK>;
     const int np = (int)b->h.size();
@@ -782,21 +804,25 @@ int enqueue_bf(rjk_batch* b, hipEvent_t* ev) {
     mark(3);
     hipLaunchKernelGGL(k_b_bf_prep<S>, dim3(grid_elem(b->t_prep.back())), dim3(256), 0, st, b->d_prob, np, tb);
     mark(4);
-    if (b->t_sub_items[0])
-        hipLaunchKernelGGL(k_b_bf_khalf<H0>, dim3(grid_items(b, b->t_sub_items[0], b->waves)), dim3(256), 0, st,
-                           b->d_prob, b->d_sub, b->d_suboff, b->t_sub_n[0]);
+    auto half = [&](auto kern, int g) {
+        if (b->t_sub_items[g])
+            hipLaunchKernelGGL(kern, dim3(grid_items(b, b->t_sub_items[g], b->waves)), dim3(256), 0, st, b->d_prob,
+                               b->d_sub + g * np, b->d_suboff + g * (np + 1), b->t_sub_n[g]);
+    };
+    half(k_b_bf_khalf<H0>, 0);
+    half(k_b_bf_khalf<H1>, 1);
+    half(k_b_bf_khalf<H2>, 2);
     mark(5);
-    if (b->t_sub_items[1])
-        hipLaunchKernelGGL(k_b_bf_khalf<H1>, dim3(grid_items(b, b->t_sub_items[1], b->waves)), dim3(256), 0, st,
-                           b->d_prob, b->d_sub + np, b->d_suboff + (np + 1), b->t_sub_n[1]);
+    half(k_b_bf_khalf<H3>, 3);
+    half(k_b_bf_khalf<H4>, 4);
     mark(6);
     if (b->t_syrk_items[0])
         hipLaunchKernelGGL(k_b_bf_ksyrk<Y0>, dim3(grid_items(b, b->t_syrk_items[0], b->waves)), dim3(Y0::NT), 0, st,
-                           b->d_prob, b->d_sub, b->d_syrkoff, b->t_sub_n[0]);
+                           b->d_prob, b->d_ssub, b->d_syrkoff, b->t_ssub_n[0]);
     mark(7);
     if (b->t_syrk_items[1])
         hipLaunchKernelGGL(k_b_bf_ksyrk<Y1>, dim3(grid_items(b, b->t_syrk_items[1], b->waves)), dim3(Y1::NT), 0, st,
-                           b->d_prob, b->d_sub + np, b->d_syrkoff + (np + 1), b->t_sub_n[1]);
+                           b->d_prob, b->d_ssub + np, b->d_syrkoff + (np + 1), b->t_ssub_n[1]);
     mark(8);
     hipLaunchKernelGGL(k_b_kreduce, dim3(grid_elem(b->t_pairs.back())), dim3(256), 0, st, b->d_prob, np, tb, b->sk);
     mark(9);
@@ -980,8 +1006,9 @@ int rjk_batch_create(int device, size_t nprob, const size_t* nbf, const size_t* 
     for (auto* t : {&b->t_pairs, &b->t_cols, &b->t_jseg, &b->t_prep, &b->t_syrk}) tab.insert(tab.end(), t->begin(), t->end());
     if (hipMalloc((void**)&b->d_prob, nprob * sizeof(BProb)) != hipSuccess ||
         hipMalloc((void**)&b->d_tab, tab.size() * sizeof(size_t)) != hipSuccess ||
-        hipMalloc((void**)&b->d_sub, 2 * nprob * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&b->d_suboff, 2 * (nprob + 1) * sizeof(size_t)) != hipSuccess ||
+        hipMalloc((void**)&b->d_sub, kHalfGroups * nprob * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&b->d_suboff, kHalfGroups * (nprob + 1) * sizeof(size_t)) != hipSuccess ||
+        hipMalloc((void**)&b->d_ssub, 2 * nprob * sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&b->d_syrkoff, 2 * (nprob + 1) * sizeof(size_t)) != hipSuccess)
         return fail_free(RJK_ERR_NOMEM);
     if (hipMemcpyAsync(b->d_prob, b->h.data(), nprob * sizeof(BProb), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
@@ -1001,7 +1028,8 @@ int rjk_batch_destroy(rjk_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
     for (void* p : b->allocs) (void)hipFree(p);
-    for (void* p : {(void*)b->d_prob, (void*)b->d_tab, (void*)b->d_sub, (void*)b->d_suboff, (void*)b->d_syrkoff,
+    for (void* p : {(void*)b->d_prob, (void*)b->d_tab, (void*)b->d_sub, (void*)b->d_suboff, (void*)b->d_ssub,
+                    (void*)b->d_syrkoff,
                     (void*)b->scratch})
         if (p) (void)hipFree(p);
     if (b->own) (void)hipStreamDestroy(b->own);
