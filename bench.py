@@ -416,7 +416,8 @@ def main_scf(args):
                                "||d eps|| <= 1e-5 (REST defaults); FP64+FP32"
                                + ("+BF16-plane" if store & jk.STORE_BF16 else "") + " B resident in HBM",
                    "mixed_prec": args.mixed_prec, "increment_prec": args.increment_prec,
-                   "pseudo_diag_err": args.pseudo_diag_err, "guard": args.guard,
+                   "pseudo_diag_err": args.pseudo_diag_err,
+                   "guard": f"energy rule at {args.guard_prec} (value); {args.guard} (value_commutator_guard)",
                    "nbf": mol.nbf, "naux": mol.naux, "nocc": mol.nelectron // 2},
         "roofline": roofline,
         "cpu_baseline": cpu,
