@@ -84,7 +84,7 @@ def parse():
 
 
 def cpu_threads(requested: int) -> tuple:
-    """(threads used, nproc): every host core up to 16, the CPU share of one GPU on the GPU box."""
+    """(threads used, nproc): every host core up to 16, one GPU's share of a multi-GPU host."""
     nproc = os.cpu_count() or 1
     return (requested or min(nproc, 16)), nproc
 
