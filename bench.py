@@ -303,6 +303,7 @@ def main_scf(args):
             ctrl.jk_prec_start = -1                    # FP32 throughout the low-precision phase
         g = lg.learned_scf(net, mol, r, S, H, ctrl, guard=guard, guard_prec=guard_prec)
         return g.outcome, {"guess": g.kind, "guard": guard, "e_learned": g.e_learned, "e_sad": g.e_sad,
+                           "guard_band_eh": g.band, "guard_refined_at_fp64": g.refined,
                            "err_learned": g.err_learned, "seconds": g.seconds,
                            "incremental_iterations": g.outcome.incremental_iterations,
                            "precisions": [int(x["prec"]) for x in g.outcome.log],

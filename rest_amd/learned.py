@@ -652,11 +652,23 @@ class LearnedScf:
     electrons: float            # 2 tr(PS) of the purified prediction
     seconds: dict = field(default_factory=dict)
     err_learned: float = None   # max|F D S - S D F| of the prediction's guess build
+    band: float = 0.0           # "energy": kept only when e_learned < e_sad - band (the estimates' error bound)
+    refined: bool = False       # "energy": e_sad was computed again at FP64 (the first estimate was inconclusive)
+
+
+# Relative bound of an energy's error when its J/K ran at this precision: used by the energy guard to
+# decide on estimates.  Measured at (H2O)64 (E about 4859 Eh): E(D1) of SAD off by 0.73 Eh at BF16X1
+# (1.5e-4 relative), 1.2e-3 Eh at BF16X2 (2.5e-7); the bounds keep a factor of two at BF16X1 and four
+# at BF16X2.  An estimate outside its bound can only keep or drop a prediction wrongly near the bound;
+# the SCF from either start converges to the same energy.
+GUARD_BAND_REL = {jk.PREC_FP64: 0.0, jk.PREC_FP32: 1e-6, jk.PREC_BF16X3: 1e-6, jk.PREC_BF16X2: 1e-6,
+                  jk.PREC_BF16X1: 3e-4}
 
 
 def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
                 mcweeny_iterations: int = 6, guard: str = "error",
-                guess_err_max: float = GUESS_ERR_MAX, guard_prec: int = None) -> LearnedScf:
+                guess_err_max: float = GUESS_ERR_MAX, guard_prec: int = None,
+                guard_band_rel: dict = None) -> LearnedScf:
     """RHF from the learned guess, with the acceptance rule folded into the SCF.
 
     The SCF starts from the purified prediction; its guess Fock build F(D_pred) decides, before
@@ -667,9 +679,12 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
         max|F D S - S D F| is below guess_err_max; the test costs two GEMMs.
       guard "energy": SAD's E(D1) is computed first (REST's guess step from the SAD density plus
         the energy of the new orbitals' density, at guard_prec, by default the SCF's starting
-        precision); kept when E(D_pred) is below it.  Both are energies of idempotent densities with
-        the molecule's electron count, so both bound the converged energy from above and compare
-        fairly.
+        precision); kept when E(D_pred), from the SCF's guess build at its starting precision, is
+        below it.  Both are energies of idempotent densities with the molecule's electron count, so
+        both bound the converged energy from above and compare fairly.  Each estimate carries an
+        error bound (GUARD_BAND_REL of its precision, or guard_band_rel): the prediction is kept
+        when E(D_pred) lies below E(D1) by more than both bounds, dropped when it lies above by
+        more, and in between E(D1) is computed again at FP64 and decides.
     A prediction that is not finite or misses the electron count by 0.5 or more is never kept."""
     if guard not in ("error", "energy"):
         raise ValueError(f"learned_scf: guard must be 'error' or 'energy', not {guard!r}")
@@ -685,29 +700,47 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
     d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
     t3 = time.perf_counter()
     enuc = mol.nuclear_repulsion()
-    e_s = None
+    e_s, band, band_l, refined = None, 0.0, 0.0, False
+    rel = {**GUARD_BAND_REL, **(guard_band_rel or {})}
+    gp = prec0 if guard_prec is None else guard_prec
+    factors = gs.sad_factors(mol) if guard == "energy" else None
+
+    def sad_e1(prec):
+        return rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec, want_e1=True, want_outputs=False,
+                             factors=factors)[4]
+
     if guard == "energy":
-        e_s = rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec0 if guard_prec is None else guard_prec,
-                            want_e1=True, want_outputs=False, factors=gs.sad_factors(mol))[4]
+        e_s = sad_e1(gp)
+        band_l = rel[prec0] * abs(e_s)              # E(D_pred) comes from the SCF's first build, at prec0
+        band = rel[gp] * abs(e_s) + band_l
     t4 = time.perf_counter()
     ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
     out, e_l, err_l = None, float("inf"), None
     if ok:
         c2 = rs.Ctrl.from_buffer_copy(ctrl)
         if guard == "energy":
-            c2.guess_energy_max = e_s
+            c2.guess_energy_max = e_s - band
         else:
             c2.guess_err_max = guess_err_max
         out = rs.rhf(r, S, H, enuc, mol.nelectron, d_pur, c2)
         e_l, err_l = out.guess_energy, out.guess_error
         if out.guess_rejected or not np.isfinite(e_l):
             out = None
+            if guard == "energy" and np.isfinite(e_l) and e_l < e_s + band and gp != jk.PREC_FP64:
+                # inconclusive: decide on E(D1) at FP64
+                refined = True
+                e_s, band = sad_e1(jk.PREC_FP64), band_l
+                if e_l < e_s - band:
+                    c2.guess_energy_max = e_s - band
+                    out = rs.rhf(r, S, H, enuc, mol.nelectron, d_pur, c2)
+                    if out.guess_rejected:
+                        out = None
     kind = "learned" if out is not None else "sad"
     if out is None:   # SAD, with REST's guess step inside rscf_rhf
         out = rs.rhf(r, S, H, enuc, mol.nelectron, d_sad, ctrl)
     t5 = time.perf_counter()
     secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "scf": t5 - t4}
-    return LearnedScf(out, kind, e_l, e_s, ne, secs, err_l)
+    return LearnedScf(out, kind, e_l, e_s, ne, secs, err_l, band, refined)
 
 
 # ---------------------------------------------------------------------------

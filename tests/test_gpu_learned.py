@@ -169,7 +169,7 @@ def test_learned_scf_matches_fp64_and_rejects(use_model, guard):
         f"{g.outcome.energy:.10f} ({g.outcome.iterations} it, precisions {precs}, incremental "
         f"{g.outcome.incremental_iterations}) vs FP64 SAD-started {ref.energy:.10f}")
     if guard == "energy":
-        assert (g.kind == "learned") == (g.e_learned < g.e_sad)
+        assert (g.kind == "learned") == (g.e_learned < g.e_sad - g.band)
     else:
         assert g.e_sad is None and g.kind == "learned" and g.err_learned < lg.GUESS_ERR_MAX
         assert forced.kind == "sad" and forced.outcome.converged
@@ -178,3 +178,38 @@ def test_learned_scf_matches_fp64_and_rejects(use_model, guard):
     assert precs[-1] == jk.PREC_FP64 and g.outcome.incremental_iterations >= 1
     if use_model:
         assert g.kind == "learned"
+
+
+def test_energy_guard_refines_an_inconclusive_estimate():
+    """The energy guard on an estimate of SAD's E(D1) at one BF16 plane: with its error bound widened
+    to 1% of the energy (so that the learned density falls inside the band), the guard recomputes
+    E(D1) at FP64, keeps the prediction, and the SCF ends at the FP64 SAD-started energy; with the
+    measured bound the prediction is kept as well (by the first estimate, or by FP64 if that was
+    inconclusive)."""
+    from pathlib import Path
+    from rest_amd import guess as gs
+    from rest_amd import jk, ri
+    from rest_amd import scf as rs
+    from rest_amd import workload as wl
+    say("(H2O)8 energy guard at BF16X1, widened band")
+    m = ri.Molecule.water_cluster(wl.water_cluster(5, 8))
+    S, T_, V = m.int1e()
+    H = T_ + V
+    net = lg.Net.load(Path(lg.__file__).resolve().parent / "models" / "learned_guess_h2o.npz")
+    with jk.DeviceRI(m.nbf, m.naux) as r:
+        m.build_rimatr(r, 1e-8, jk.STORE_FP64 | jk.STORE_FP32 | jk.STORE_BF16)
+        ref = rs.rhf(r, S, H, m.nuclear_repulsion(), m.nelectron, gs.sad_guess(m, r, S, H).d1, rs.default_ctrl())
+        ctrl = rs.mixed_ctrl()
+        wide = lg.learned_scf(net, m, r, S, H, ctrl, guard="energy", guard_prec=jk.PREC_BF16X1,
+                              guard_band_rel={jk.PREC_BF16X1: 1e-2})
+        plain = lg.learned_scf(net, m, r, S, H, ctrl, guard="energy", guard_prec=jk.PREC_BF16X1)
+        e1_64 = rs.guess_step(r, S, H, m.nuclear_repulsion(), (m.nelectron,), sum(gs.sad_density(m)),
+                              jk.PREC_FP64, want_e1=True, want_outputs=False, factors=gs.sad_factors(m))[4]
+    say(f"  widened: kind {wide.kind}, refined {wide.refined}, E_learned {wide.e_learned:.6f}, E_sad {wide.e_sad:.6f} "
+        f"(FP64 {e1_64:.6f}); measured band: kind {plain.kind}, refined {plain.refined}, band {plain.band:.3f}")
+    assert wide.refined and wide.kind == "learned" and wide.e_sad == pytest.approx(e1_64, abs=1e-8)
+    assert plain.kind == "learned" and plain.e_learned < plain.e_sad - plain.band
+    if plain.refined:
+        assert plain.e_sad == pytest.approx(e1_64, abs=1e-8)
+    for g in (wide, plain):
+        assert g.outcome.converged and abs(g.outcome.energy - ref.energy) <= 1e-8
