@@ -45,6 +45,7 @@ import ctypes as C
 import json
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Optional
@@ -694,11 +695,6 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
     da, db = gs.sad_density(mol)
     d_sad = da + db
     t1 = time.perf_counter()
-    d_pred = predict_density(net, mol, spherical_average(mol, d_sad))
-    d_pred *= mol.nelectron / float(np.sum(d_pred * S))
-    t2 = time.perf_counter()
-    d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
-    t3 = time.perf_counter()
     enuc = mol.nuclear_repulsion()
     e_s, band, band_l, refined = None, 0.0, 0.0, False
     rel = {**GUARD_BAND_REL, **(guard_band_rel or {})}
@@ -709,11 +705,30 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
         return rs.guess_step(r, S, H, enuc, (mol.nelectron,), d_sad, prec, want_e1=True, want_outputs=False,
                              factors=factors)[4]
 
+    def predict():
+        a = time.perf_counter()
+        d = predict_density(net, mol, spherical_average(mol, d_sad))
+        d *= mol.nelectron / float(np.sum(d * S))
+        return d, time.perf_counter() - a
+
     if guard == "energy":
-        e_s = sad_e1(gp)
+        # the prediction (host descriptors and decode, the network on its own stream) runs beside
+        # SAD's guess step and E(D1) on the J/K context
+        with ThreadPoolExecutor(max_workers=1) as ex:
+            fut = ex.submit(predict)
+            a = time.perf_counter()
+            e_s = sad_e1(gp)
+            t_e1 = time.perf_counter() - a
+            d_pred, t_pred = fut.result()
         band_l = rel[prec0] * abs(e_s)              # E(D_pred) comes from the SCF's first build, at prec0
         band = rel[gp] * abs(e_s) + band_l
-    t4 = time.perf_counter()
+    else:
+        d_pred, t_pred = predict()
+        t_e1 = 0.0
+    t2 = time.perf_counter()
+    d_pur, ne = rs.mcweeny(r, S, d_pred, mcweeny_iterations)
+    t3 = time.perf_counter()
+    t4 = t3
     ok = bool(np.all(np.isfinite(d_pur))) and abs(ne - mol.nelectron) < 0.5
     out, e_l, err_l = None, float("inf"), None
     if ok:
@@ -739,7 +754,8 @@ def learned_scf(net: Net, mol: "ri.Molecule", r: "jk.DeviceRI", S, H, ctrl=None,
     if out is None:   # SAD, with REST's guess step inside rscf_rhf
         out = rs.rhf(r, S, H, enuc, mol.nelectron, d_sad, ctrl)
     t5 = time.perf_counter()
-    secs = {"sad": t1 - t0, "predict": t2 - t1, "mcweeny": t3 - t2, "sad_energy": t4 - t3, "scf": t5 - t4}
+    secs = {"sad": t1 - t0, "predict": t_pred, "sad_energy": t_e1, "predict_and_sad_energy": t2 - t1,
+            "mcweeny": t3 - t2, "scf": t5 - t4}
     return LearnedScf(out, kind, e_l, e_s, ne, secs, err_l, band, refined)
 
 
