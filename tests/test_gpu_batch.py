@@ -317,10 +317,22 @@ def test_batched_scf_matches_single_problem_scf():
     st = {}
     outs = bs.rhf_batch(mols, stats=st, budget_bytes=8 << 30)
     say(f"  {st}")
+    refs = []
     for m, o in zip(mols, outs):
         ref, _ = rs.run_molecule(m, guess="sad")
+        refs.append(ref)
         say(f"  nbf {m.nbf}: batched {o.energy:.10f} ({o.iterations} it, batched={o.batched}), "
             f"single {ref.energy:.10f} ({ref.iterations} it)")
-        assert o.converged and ref.converged
+        assert o.converged and ref.converged and o.batched
         assert abs(o.energy - ref.energy) <= 1e-9
         assert abs(np.trace(o.dm @ m.int1e()[0]) - m.nelectron) <= 1e-8
+    # linear density mixing before DIIS needs more K columns than the batch holds: those molecules
+    # leave the batch and finish on the single-problem SCF with the same controls
+    ctrl = rs.default_ctrl()
+    ctrl.start_diis_cycle = 3
+    st = {}
+    outs = bs.rhf_batch(mols[:2], ctrl=ctrl, stats=st, budget_bytes=8 << 30)
+    say(f"  start_diis_cycle 3: {st['fallbacks']} fallbacks")
+    assert st["fallbacks"] == 2
+    for m, o, ref in zip(mols[:2], outs, refs[:2]):
+        assert not o.batched and o.converged and abs(o.energy - ref.energy) <= 1e-7
