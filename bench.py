@@ -66,8 +66,9 @@ def parse():
                     help="jk mode: fp64 plain path; fp32: FP32 copy of B, FP32 MFMA K, FP64 accumulation of J; "
                          "bf16x3 / bf16x2 / bf16x1: K on three / two / one BF16 planes of B (6 / 3 / 1 MFMA products), "
                          "J on the FP32 copy")
-    ap.add_argument("--mixed-prec", default="bf16x3", choices=["fp32", "bf16x3"],
-                    help="scf mode: J/K precision of the mixed SCF before its switch to FP64")
+    ap.add_argument("--mixed-prec", default="bf16x3", choices=["fp32", "bf16x2", "bf16x3"],
+                    help="scf mode: J/K precision of the mixed SCF before its switch to FP64 (bf16x3: two planes "
+                         "until max|e| < 1e-4, then three; bf16x2: two planes throughout)")
     ap.add_argument("--pseudo-diag-err", type=float, default=1e-2,
                     help="scf mode: low-precision iterations with max|e| below this pseudo-diagonalise (0: never)")
     ap.add_argument("--increment-prec", default="bf16x2", choices=["none", "fp32", "bf16x2", "bf16x3"],
