@@ -1071,7 +1071,7 @@ static int stage_b(rjk_batch* b, size_t i, const double* host_compact, uint64_t 
 }
 
 int rjk_batch_set_rimatr(rjk_batch* b, size_t i, const double* rimatr) {
-    if (!b || !rimatr || i >= b->h.size()) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_set_rimatr: bad argument") : 1;
+    if (!b || !rimatr || i >= b->h.size()) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_set_rimatr: bad argument") : RJK_ERR_ARG;
     return stage_b(b, i, rimatr, 0, false);
 }
 
@@ -1125,7 +1125,7 @@ int rjk_batch_set_inputs(rjk_batch* b, size_t i, const double* dm, const double*
 }
 
 int rjk_batch_get_outputs(rjk_batch* b, size_t i, double* vj, double* vk) {
-    if (!b || i >= b->h.size()) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_get_outputs: bad argument") : 1;
+    if (!b || i >= b->h.size()) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_get_outputs: bad argument") : RJK_ERR_ARG;
     const BProb& P = b->h[i];
     BHIP(b, hipSetDevice(b->device));
     if (vj) BHIP(b, hipMemcpyAsync(vj, P.vj, P.npair * 8, hipMemcpyDeviceToHost, b->stream));
@@ -1177,7 +1177,7 @@ int rjk_batch_synchronize(rjk_batch* b) {
 }
 
 int rjk_batch_autotune(rjk_batch* b, int reps, double* best_ms) {
-    if (!b || reps < 1) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_autotune: reps must be >= 1") : 1;
+    if (!b || reps < 1) return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_autotune: reps must be >= 1") : RJK_ERR_ARG;
     BHIP(b, hipSetDevice(b->device));
     drop_graph(b);
     const int cuts[] = {0, 256, 512, 1 << 30};
@@ -1233,7 +1233,7 @@ int rjk_batch_config(const rjk_batch* b, int* khalf_cut, int* waves_per_cu) {
 
 int rjk_batch_set_config(rjk_batch* b, int khalf_cut, int waves_per_cu) {
     if (!b || waves_per_cu < 1 || waves_per_cu > 16 || khalf_cut < 0)
-        return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_set_config: waves_per_cu must be 1..16, cut >= 0") : 1;
+        return b ? bfail(b, RJK_ERR_ARG, "rjk_batch_set_config: waves_per_cu must be 1..16, cut >= 0") : RJK_ERR_ARG;
     b->khalf_cut = khalf_cut;
     b->waves = waves_per_cu;
     drop_graph(b);
