@@ -64,7 +64,7 @@ struct BProb {
 // Prefix tables of the work lists (nprob + 1 entries each, device).
 struct Tables {
     const size_t* pairs;   // Σ npair
-    const size_t* cols;    // Σ naux
+    const size_t* cols;    // Σ ceil(naux / kGammaGroup): γ work items
     const size_t* jseg;    // Σ J-accumulation segments
     const size_t* prep;    // Σ nbf_pad·nk
     const size_t* syrk;    // Σ nsplit·ntile2
@@ -88,15 +88,6 @@ __device__ inline void pair_ij(size_t p, size_t* i, size_t* j) {
     *i = p - tri(jj);
 }
 
-__device__ inline double block_sum256(double v, double* sh) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    const double s = sh[0] + sh[1] + sh[2] + sh[3];
-    __syncthreads();
-    return s;
-}
-
 // ---- J -----------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_b_weights(const BProb* __restrict__ pr, int np, Tables tb) {
     const size_t total = tb.pairs[np];
@@ -107,8 +98,8 @@ __global__ __launch_bounds__(256) void k_b_weights(const BProb* __restrict__ pr,
         const size_t p = t - tb.pairs[q];
         size_t i, j;
         pair_ij(p, &i, &j);
-        const double d = P.dm[i + j * (size_t)P.nbf];   // upper triangle (row <= column)
-        P.w[p] = i == j ? d : 2.0 * d;
+        const double d = glb(P.dm)[i + j * (size_t)P.nbf];   // upper triangle (row <= column)
+        glb(P.w)[p] = i == j ? d : 2.0 * d;
     }
 }
 
@@ -117,27 +108,54 @@ struct Vec16;
 template <> struct Vec16<double> { typedef double v __attribute__((ext_vector_type(2))); static constexpr int n = 2; };
 template <> struct Vec16<float> { typedef float v __attribute__((ext_vector_type(4))); static constexpr int n = 4; };
 
-// γ_P for one (problem, auxiliary column) per work item
+// γ_P for one (problem, group of kGammaGroup auxiliary columns) per work item: each thread loads
+// its weights once and applies them to the group's columns of B (streamed with non-temporal loads)
+constexpr int kGammaGroup = 8;
 template <typename T>
 __global__ __launch_bounds__(256) void k_b_gamma(const BProb* __restrict__ pr, int np, Tables tb) {
     using V = typename Vec16<T>::v;
-    constexpr int NV = Vec16<T>::n;
-    __shared__ double sh[4];
+    using W = typename Vec16<double>::v;
+    constexpr int NV = Vec16<T>::n, G = kGammaGroup;
+    __shared__ double sh[4][G];
     const size_t total = tb.cols[np];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (size_t it = blockIdx.x; it < total; it += gridDim.x) {
         const int q = find_prob(tb.cols, np, it);
         const BProb& P = pr[q];
         if (!P.active) continue;
-        const size_t col = it - tb.cols[q];
-        const T* __restrict__ bc = (const T*)P.b + col * P.ldb;
-        double acc = 0.0;
-        for (size_t p = (size_t)threadIdx.x * NV; p < P.ldb; p += 256 * NV) {
-            const V bv = *(const V*)(bc + p);
+        const int c0 = (int)(it - tb.cols[q]) * G, cnt = min(G, P.naux - c0);
+        const RJK_GLB T* __restrict__ bc = glb((const T*)P.b) + (size_t)c0 * P.ldb;
+        const RJK_GLB double* __restrict__ w = glb((const double*)P.w);
+        double acc[G];
 #pragma unroll
-            for (int e = 0; e < NV; ++e) acc += (double)bv[e] * P.w[p + e];
+        for (int c = 0; c < G; ++c) acc[c] = 0.0;
+        for (size_t p = (size_t)threadIdx.x * NV; p < P.ldb; p += 256 * NV) {
+            double wl[NV];
+#pragma unroll
+            for (int h = 0; h < NV / 2; ++h) {
+                const W t = *(const RJK_GLB W*)(w + p + 2 * h);
+                wl[2 * h] = t[0];
+                wl[2 * h + 1] = t[1];
+            }
+#pragma unroll
+            for (int c = 0; c < G; ++c)
+                if (c < cnt) {
+                    const V bv = __builtin_nontemporal_load((const RJK_GLB V*)(bc + (size_t)c * P.ldb + p));
+#pragma unroll
+                    for (int e = 0; e < NV; ++e) acc[c] = fma((double)bv[e], wl[e], acc[c]);
+                }
         }
-        const double s = block_sum256(acc, sh);
-        if (threadIdx.x == 0) P.gamma[col] = s;
+#pragma unroll
+        for (int c = 0; c < G; ++c) {
+            double v = acc[c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) sh[wave][c] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < cnt)
+            glb(P.gamma)[c0 + threadIdx.x] =
+                sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+        __syncthreads();
     }
 }
 
@@ -156,28 +174,29 @@ __global__ __launch_bounds__(256) void k_b_jacc(const BProb* __restrict__ pr, in
         double acc[NV];
 #pragma unroll
         for (int e = 0; e < NV; ++e) acc[e] = 0.0;
-        const T* __restrict__ bp = (const T*)P.b + p0;
+        const RJK_GLB T* __restrict__ bp = glb((const T*)P.b) + p0;
+        const RJK_GLB double* __restrict__ gam = glb((const double*)P.gamma);
         int col = 0;
         for (; col + 4 <= P.naux; col += 4) {
             V v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = *(const V*)(bp + (size_t)(col + u) * P.ldb);
+            for (int u = 0; u < 4; ++u) v[u] = *(const RJK_GLB V*)(bp + (size_t)(col + u) * P.ldb);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const double g = P.gamma[col + u];
+                const double g = gam[col + u];
 #pragma unroll
                 for (int e = 0; e < NV; ++e) acc[e] += (double)v[u][e] * g;
             }
         }
         for (; col < P.naux; ++col) {
-            const V v = *(const V*)(bp + (size_t)col * P.ldb);
-            const double g = P.gamma[col];
+            const V v = *(const RJK_GLB V*)(bp + (size_t)col * P.ldb);
+            const double g = gam[col];
 #pragma unroll
             for (int e = 0; e < NV; ++e) acc[e] += (double)v[e] * g;
         }
 #pragma unroll
         for (int e = 0; e < NV; ++e)
-            if (p0 + e < P.npair) P.vj[p0 + e] = scale * acc[e];
+            if (p0 + e < P.npair) glb(P.vj)[p0 + e] = scale * acc[e];
     }
 }
 
@@ -192,11 +211,11 @@ __global__ __launch_bounds__(256) void k_b_prep(const BProb* __restrict__ pr, in
         const size_t e = t - tb.prep[q];
         const int c = (int)(e % P.nk), nu = (int)(e / P.nk);
         double v = 0.0;
-        if (c < P.nocc && nu < P.nbf) v = P.c[(size_t)c * P.nbf + nu] * sqrt(fabs(P.occ[c]));
-        ((T*)P.cocc)[e] = (T)v;
+        if (c < P.nocc && nu < P.nbf) v = glb(P.c)[(size_t)c * P.nbf + nu] * sqrt(fabs(glb(P.occ)[c]));
+        glb((T*)P.cocc)[e] = (T)v;
         if (nu == 0) {
-            const double o = c < P.nocc ? P.occ[c] : 0.0;
-            ((T*)P.sign)[c] = (T)(o > 0 ? 1.0 : (o < 0 ? -1.0 : 0.0));
+            const double o = c < P.nocc ? glb(P.occ)[c] : 0.0;
+            glb((T*)P.sign)[c] = (T)(o > 0 ? 1.0 : (o < 0 ? -1.0 : 0.0));
         }
     }
 }
@@ -218,8 +237,8 @@ __global__ __launch_bounds__(kBlock) void k_b_khalf(const BProb* __restrict__ pr
         const int Pl = (int)((kq / per) * kXcd + xcd);
         if (Pl >= P.naux) continue;   // padding items (uniform per work-group)
         const int rem = (int)(kq % per), mt = rem / ntc, ct = rem % ntc;
-        khalf_tile<T, FM, FN>((const T*)P.b + (size_t)Pl * P.ldb, P.nbf, mt * TM, ct * TN, (const T*)P.cocc, P.nk,
-                              P.nbf_pad, (T*)P.m + (size_t)Pl * P.nk * P.nbf_pad);
+        khalf_tile<T, FM, FN>(glb((const T*)P.b) + (size_t)Pl * P.ldb, P.nbf, mt * TM, ct * TN, glb((const T*)P.cocc),
+                              P.nk, P.nbf_pad, glb((T*)P.m) + (size_t)Pl * P.nk * P.nbf_pad);
     }
 }
 
@@ -239,8 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_b_ksyrk(const BProb* __restrict__ pr
         size_t tm, tn;
         upper_tile(t, &tm, &tn);
         const int Pb = split * P.pps, Pe = min(P.naux, Pb + P.pps);
-        ksyrk_tile<T, F>((const T*)P.m, P.nbf_pad, P.nk, (int)tm * Tt, (int)tn * Tt, 0, P.ncb, Pb, Pe,
-                         (const T*)P.sign, 0, P.kpart + (size_t)split * P.nbf_pad * P.nbf_pad);
+        ksyrk_tile<T, F>(glb((const T*)P.m), P.nbf_pad, P.nk, (int)tm * Tt, (int)tn * Tt, 0, P.ncb, Pb, Pe,
+                         glb((const T*)P.sign), 0, glb(P.kpart) + (size_t)split * P.nbf_pad * P.nbf_pad);
     }
 }
 
@@ -255,8 +274,8 @@ __global__ __launch_bounds__(256) void k_b_kreduce(const BProb* __restrict__ pr,
         pair_ij(p, &i, &j);
         const size_t slot = (size_t)P.nbf_pad * P.nbf_pad;
         double acc = 0.0;
-        for (int s = 0; s < P.nsplit; ++s) acc += P.kpart[s * slot + j * P.nbf_pad + i];
-        P.vk[p] = scale * acc;
+        for (int s = 0; s < P.nsplit; ++s) acc += glb((const double*)P.kpart)[s * slot + j * P.nbf_pad + i];
+        glb(P.vk)[p] = scale * acc;
     }
 }
 
@@ -293,13 +312,13 @@ __global__ __launch_bounds__(256) void k_b_bf_prep(const BProb* __restrict__ pr,
         if (!P.active) continue;
         const size_t e = t - tb.prep[q], plane = (size_t)P.nk * P.nbf_pad;
         const int c = (int)(e / P.nbf_pad), nu = (int)(e % P.nbf_pad);
-        const double w = c < P.nocc ? P.occ[c] : 0.0;
-        const double v = (c < P.nocc && nu < P.nbf) ? P.c[(size_t)c * P.nbf + nu] * sqrt(fabs(w)) : 0.0;
+        const double w = c < P.nocc ? glb(P.occ)[c] : 0.0;
+        const double v = (c < P.nocc && nu < P.nbf) ? glb(P.c)[(size_t)c * P.nbf + nu] * sqrt(fabs(w)) : 0.0;
         __bf16 h[S];
         split_to<S>(v, h);
 #pragma unroll
-        for (int k = 0; k < S; ++k) ((__bf16*)P.cocc)[k * plane + e] = h[k];
-        if (nu == 0) ((unsigned short*)P.sign)[c] = w < 0 ? 0x8000 : 0;
+        for (int k = 0; k < S; ++k) glb((__bf16*)P.cocc)[k * plane + e] = h[k];
+        if (nu == 0) glb((unsigned short*)P.sign)[c] = w < 0 ? 0x8000 : 0;
     }
 }
 
@@ -320,9 +339,9 @@ struct BfHalf {
 };
 
 template <class H>
-__device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const float* __restrict__ bcol, int nbf,
-                                             int mu0, int c0, const __bf16* __restrict__ ct, size_t pct,
-                                             int nbf_pad, int nk, __bf16* __restrict__ mP, size_t pm) {
+__device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const RJK_GLB float* __restrict__ bcol, int nbf,
+                                             int mu0, int c0, const RJK_GLB __bf16* __restrict__ ct, size_t pct,
+                                             int nbf_pad, int nk, RJK_GLB __bf16* __restrict__ mP, size_t pm) {
     using namespace bft;
     constexpr int S = H::S_, NP = npairs<S>(), TK = H::TK_, NPC = H::NPC;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -357,7 +376,7 @@ __device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const flo
 #pragma unroll
             for (int j = 0; j < H::LX; ++j) {
                 const int e = tid + j * H::NT;
-                rx[k][j] = *(const u4*)(ct + k * pct + (size_t)(c0 + e / NPC) * nbf_pad + nu0 + (e % NPC) * 8);
+                rx[k][j] = ld16(ct + k * pct + (size_t)(c0 + e / NPC) * nbf_pad + nu0 + (e % NPC) * 8);
             }
         below = mu0 >= nu0 + TK - 1;
 #pragma unroll
@@ -451,7 +470,7 @@ __device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const flo
             for (int r = 0; r < 4; ++r) split_to_f<S>(acc[i][j][r] + acc2[i][j][r], h[r]);
 #pragma unroll
             for (int k = 0; k < S; ++k)
-                *(bft::bf4*)(mP + (size_t)k * pm + (size_t)mu * nk + c) = bft::bf4{h[0][k], h[1][k], h[2][k], h[3][k]};
+                *(RJK_GLB bft::bf4*)(mP + (size_t)k * pm + (size_t)mu * nk + c) = bft::bf4{h[0][k], h[1][k], h[2][k], h[3][k]};
         }
 }
 
@@ -472,9 +491,9 @@ __global__ __launch_bounds__(256) void k_b_bf_khalf(const BProb* __restrict__ pr
         if (Pl >= P.naux) continue;   // padding items (uniform per work-group)
         const int rem = (int)(kq % per), mt = rem / ntc, ctile = rem % ntc;
         const size_t pm = (size_t)P.naux * P.nbf_pad * P.nk;
-        bf_half_tile<H>(lds, (const float*)P.b + (size_t)Pl * P.ldb, P.nbf, mt * H::WGY, ctile * H::WGX,
-                        (const __bf16*)P.cocc, (size_t)P.nk * P.nbf_pad, P.nbf_pad, P.nk,
-                        (__bf16*)P.m + (size_t)Pl * P.nbf_pad * P.nk, pm);
+        bf_half_tile<H>(lds, glb((const float*)P.b) + (size_t)Pl * P.ldb, P.nbf, mt * H::WGY, ctile * H::WGX,
+                        glb((const __bf16*)P.cocc), (size_t)P.nk * P.nbf_pad, P.nbf_pad, P.nk,
+                        glb((__bf16*)P.m) + (size_t)Pl * P.nbf_pad * P.nk, pm);
     }
 }
 
@@ -501,8 +520,8 @@ __global__ __launch_bounds__(C::NT) void k_b_bf_ksyrk(const BProb* __restrict__ 
         const int Pb = split * P.pps, Pe = min(P.naux, Pb + P.pps);
         const int nq = (Pe > Pb ? Pe - Pb : 0) * ncb;
         const size_t mplane = (size_t)P.naux * P.nbf_pad * P.nk, mPs = (size_t)P.nbf_pad * P.nk;
-        const __bf16* mpl = (const __bf16*)P.m;
-        const unsigned short* smask = (const unsigned short*)P.sign;
+        const RJK_GLB __bf16* mpl = glb((const __bf16*)P.m);
+        const RJK_GLB unsigned short* smask = glb((const unsigned short*)P.sign);
         bft::f4 acc[C::FX][C::FY], acc2[C::FX][C::FY];   // leading product; corrections
         bft::nt_tile2<C, true>(
             lds, [&](int q) { return mpl + (Pb + q / ncb) * mPs + (size_t)nu0 * P.nk + (q % ncb) * C::TK; },
@@ -510,13 +529,13 @@ __global__ __launch_bounds__(C::NT) void k_b_bf_ksyrk(const BProb* __restrict__ 
             [&](int q) { return mpl + (Pb + q / ncb) * mPs + (size_t)mu0 * P.nk + (q % ncb) * C::TK; },
             (size_t)P.nk, mplane,
             [&](int q, int pc) {
-                const unsigned short* sm = smask + (q % ncb) * C::TK + pc;
+                const RJK_GLB unsigned short* sm = smask + (q % ncb) * C::TK + pc;
                 return bft::u4{(unsigned)sm[0] | ((unsigned)sm[1] << 16), (unsigned)sm[2] | ((unsigned)sm[3] << 16),
                                (unsigned)sm[4] | ((unsigned)sm[5] << 16), (unsigned)sm[6] | ((unsigned)sm[7] << 16)};
             },
             nq, acc, acc2);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wx = wave % C::WX, wy = wave / C::WX;
-        double* __restrict__ kp = P.kpart + (size_t)split * P.nbf_pad * P.nbf_pad;
+        RJK_GLB double* __restrict__ kp = glb(P.kpart) + (size_t)split * P.nbf_pad * P.nbf_pad;
 #pragma unroll
         for (int i = 0; i < C::FX; ++i)
 #pragma unroll
@@ -995,7 +1014,7 @@ int rjk_batch_create(int device, size_t nprob, const size_t* nbf, const size_t* 
         for (const BProb& P : b->h) t.push_back(t.back() + f(P));
     };
     prefix(b->t_pairs, [](const BProb& P) { return P.npair; });
-    prefix(b->t_cols, [](const BProb& P) { return (size_t)P.naux; });
+    prefix(b->t_cols, [](const BProb& P) { return (size_t)(P.naux + kGammaGroup - 1) / kGammaGroup; });
     prefix(b->t_jseg, [&](const BProb& P) { return (P.ldb + 256 * (16 / es) - 1) / (256 * (16 / es)); });
     prefix(b->t_prep, [](const BProb& P) { return (size_t)P.nbf_pad * P.nk; });
     prefix(b->t_syrk, [](const BProb& P) {

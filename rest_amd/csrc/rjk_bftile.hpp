@@ -59,6 +59,12 @@ struct Cfg {
 
 __device__ inline f4 mfma(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
+// 16-byte load that keeps the pointer's address space (a global pointer stays a global load)
+__device__ __forceinline__ u4 ld16(const __bf16* p) { return *(const u4*)p; }
+__device__ __forceinline__ u4 ld16(const __attribute__((address_space(1))) __bf16* p) {
+    return *(const __attribute__((address_space(1))) u4*)p;
+}
+
 // Runs nq k-blocks.  xaddr(q) / yaddr(q): pointer to row 0, plane 0, first element of
 // k-block q; ldx / ldy row strides, px / py plane strides (elements); xmask(q, col):
 // the 8 sign masks (0x8000 flips) of the X piece at k-block column col, as a u4 of
@@ -88,16 +94,14 @@ __device__ __forceinline__ void nt_tile2(__bf16* __restrict__ lds, XA xaddr, siz
         }
     u4 rx[S][C::LX], ry[S][C::LY], rm;
     auto fetch = [&](int q) {
-        const __bf16* xb = xaddr(q);
-        const __bf16* yb = yaddr(q);
+        const auto xb = xaddr(q);
+        const auto yb = yaddr(q);
 #pragma unroll
         for (int k = 0; k < S; ++k) {
 #pragma unroll
-            for (int j = 0; j < C::LX; ++j)
-                rx[k][j] = *(const u4*)(xb + (size_t)k * px + (size_t)(row + j * RSTEP) * ldx + pc);
+            for (int j = 0; j < C::LX; ++j) rx[k][j] = ld16(xb + (size_t)k * px + (size_t)(row + j * RSTEP) * ldx + pc);
 #pragma unroll
-            for (int j = 0; j < C::LY; ++j)
-                ry[k][j] = *(const u4*)(yb + (size_t)k * py + (size_t)(row + j * RSTEP) * ldy + pc);
+            for (int j = 0; j < C::LY; ++j) ry[k][j] = ld16(yb + (size_t)k * py + (size_t)(row + j * RSTEP) * ldy + pc);
         }
         rm = xmask(q, pc);
     };

@@ -26,6 +26,14 @@ constexpr int kXcd = 8;               // MI355X: 8 XCDs, work-groups dealt round
 
 __device__ inline size_t tri(size_t j) { return j * (j + 1) / 2; }
 
+// A device pointer read from a table in memory (not a kernel argument) is a flat pointer to the
+// compiler, and flat loads count against the LDS wait counter as well as the vector-memory one,
+// so every LDS wait of a pipelined tile would also wait for the next block's loads.  glb() gives
+// such a pointer the global address space; the tiles below take their pointers as template types,
+// so these accesses stay global loads and stores.
+#define RJK_GLB __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ RJK_GLB T* glb(T* p) { return (RJK_GLB T*)p; }
+
 // One 16x16x4 MFMA per element type.  Lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; register r of lane l holds D[row(l, r)][l%16] (probed on
 // gfx950: FP64 interleaves rows by 4, FP32 keeps 4 consecutive rows).
@@ -62,9 +70,9 @@ __device__ inline void upper_tile(size_t t, size_t* tm, size_t* tn) {
 //   registers while the MFMAs consume the current one from LDS; LDS is
 //   double-buffered, one barrier per k-block.
 // ---------------------------------------------------------------------------
-template <typename T, int FM, int FN>
-__device__ __forceinline__ void khalf_tile(const T* __restrict__ bcol, int nbf, int mu0, int c0,
-                                           const T* __restrict__ cocc, int nk, int ldm, T* __restrict__ mP) {
+template <typename T, int FM, int FN, typename BP, typename CP, typename MP>
+__device__ __forceinline__ void khalf_tile(BP __restrict__ bcol, int nbf, int mu0, int c0, CP __restrict__ cocc,
+                                           int nk, int ldm, MP __restrict__ mP) {
     using MM = Mma<T>;
     constexpr int TM = 2 * FM * 16, TN = 2 * FN * 16;
     constexpr int NA = TK * TM / kBlock, NB = TK * TN / kBlock;
@@ -183,10 +191,9 @@ struct SyrkCols {
     int off[2], cnt[2];
 };
 
-template <typename T, int F>
-__device__ __forceinline__ void ksyrk_tile(const T* __restrict__ m, int ldm, int nk, int mu0, int nu0, int cb, int ncb,
-                                           int Pb, int Pe, const T* __restrict__ sign, int beta,
-                                           double* __restrict__ kp) {
+template <typename T, int F, typename MP, typename SP, typename KP>
+__device__ __forceinline__ void ksyrk_tile(MP __restrict__ m, int ldm, int nk, int mu0, int nu0, int cb, int ncb,
+                                           int Pb, int Pe, SP __restrict__ sign, int beta, KP __restrict__ kp) {
     using MM = Mma<T>;
     constexpr int Tt = 2 * F * 16;
     constexpr int NL = TK * Tt / kBlock;
@@ -202,7 +209,7 @@ __device__ __forceinline__ void ksyrk_tile(const T* __restrict__ m, int ldm, int
     // fragments), so the loads stay in flight across the current block's MFMAs
     auto fetch = [&](int q) {
         const int P = Pb + q / ncb, c = cb + (q % ncb) * TK;
-        const T* __restrict__ base = m + ((size_t)P * nk + c) * ldm;
+        const auto base = m + ((size_t)P * nk + c) * ldm;
 #pragma unroll
         for (int r = 0; r < NL; ++r) {
             const int e = tid + r * kBlock, mm = e % Tt, k = e / Tt;
@@ -264,7 +271,7 @@ __device__ __forceinline__ void ksyrk_tile(const T* __restrict__ m, int ldm, int
             const int nb = nu0 + wn * F * 16 + j * 16;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                double* dst = kp + (size_t)(nb + MM::row(lane, r)) * ldm + mu;
+                const auto dst = kp + (size_t)(nb + MM::row(lane, r)) * ldm + mu;
                 double v = (double)acc[i][j][r];
                 if (beta) v += *dst;
                 *dst = v;
