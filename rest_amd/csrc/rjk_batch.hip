@@ -176,20 +176,22 @@ __global__ __launch_bounds__(256) void k_b_jacc(const BProb* __restrict__ pr, in
         for (int e = 0; e < NV; ++e) acc[e] = 0.0;
         const RJK_GLB T* __restrict__ bp = glb((const T*)P.b) + p0;
         const RJK_GLB double* __restrict__ gam = glb((const double*)P.gamma);
+        constexpr int U = 8;   // columns whose loads are in flight together
         int col = 0;
-        for (; col + 4 <= P.naux; col += 4) {
-            V v[4];
+        for (; col + U <= P.naux; col += U) {
+            V v[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = *(const RJK_GLB V*)(bp + (size_t)(col + u) * P.ldb);
+            for (int u = 0; u < U; ++u)
+                v[u] = __builtin_nontemporal_load((const RJK_GLB V*)(bp + (size_t)(col + u) * P.ldb));
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const double g = gam[col + u];
 #pragma unroll
                 for (int e = 0; e < NV; ++e) acc[e] += (double)v[u][e] * g;
             }
         }
         for (; col < P.naux; ++col) {
-            const V v = *(const RJK_GLB V*)(bp + (size_t)col * P.ldb);
+            const V v = __builtin_nontemporal_load((const RJK_GLB V*)(bp + (size_t)col * P.ldb));
             const double g = gam[col];
 #pragma unroll
             for (int e = 0; e < NV; ++e) acc[e] += (double)v[e] * g;
