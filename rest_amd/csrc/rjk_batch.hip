@@ -340,6 +340,8 @@ struct BfHalf {
     __device__ static constexpr int sw(int r) { return NPC == 4 ? ((r >> 1) & 3) : ((r >> 1) & 7); }
 };
 
+typedef float F4a __attribute__((ext_vector_type(4), aligned(4)));   // 4 floats at 4-byte alignment
+
 template <class H>
 __device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const RJK_GLB float* __restrict__ bcol, int nbf,
                                              int mu0, int c0, const RJK_GLB __bf16* __restrict__ ct, size_t pct,
@@ -387,9 +389,15 @@ __device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const RJK
             task_rp(j, &r, &p);
             const int mu = mu0 + r, nub = nu0 + p * 8;
             if (below) {
+                // 8 consecutive ν of row μ: two 16-byte loads at the row's 4-byte alignment
                 const size_t base = mu < nbf ? tri((size_t)mu) + nub : 0;
+                const RJK_GLB F4a* src = (const RJK_GLB F4a*)(bcol + base);
+                const F4a lo = src[0], hi = src[1];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) rb[j][i] = mu < nbf ? bcol[base + i] : 0.f;
+                for (int i = 0; i < 4; ++i) {
+                    rb[j][i] = mu < nbf ? lo[i] : 0.f;
+                    rb[j][4 + i] = mu < nbf ? hi[i] : 0.f;
+                }
             } else {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
