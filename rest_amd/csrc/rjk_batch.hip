@@ -465,7 +465,9 @@ __device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const RJK
         stash();
         __syncthreads();
         if (q + 1 < nq) fetch((q + 1) * TK);
+        __builtin_amdgcn_s_setprio(1);   // MFMA phase first, as in rjk_bftile.hpp
         compute();
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     }
     // M' planes: lane holds c = c0 + 16(wx FX + i) + 4(l/16) + r for μ = mu0 + 16(wy FY + j) + l%16
