@@ -465,7 +465,7 @@ __device__ __forceinline__ void bf_half_tile(__bf16* __restrict__ lds, const RJK
         stash();
         __syncthreads();
         if (q + 1 < nq) fetch((q + 1) * TK);
-        __builtin_amdgcn_s_setprio(1);   // MFMA phase first, as in rjk_bftile.hpp
+        __builtin_amdgcn_s_setprio(1);   // MFMA phase first (batch of 512 clusters: 874 -> 853 ms at BF16X3)
         compute();
         __builtin_amdgcn_s_setprio(0);
         __syncthreads();

@@ -55,6 +55,10 @@ struct Cfg {
     // both the fragment reads (16 rows, one piece) and the row-piece writes, where (r ^ r/8) % 8
     // costs 2 conflict cycles per 16-byte fragment read at 8 pieces a row (1.33 at 4)
     __device__ static constexpr int sw(int r) { return NPC == 4 ? ((r >> 1) & 3) : ((r >> 1) & 7); }
+    // MFMA phase at raised wave priority: measured faster with 64-deep k-blocks (two-plane half-
+    // transform 5.42 -> 5.08 ms, SYRK 2.65 -> 2.51 ms; three-plane half-transform 6.75 -> 6.64 ms) and
+    // slower with 32-deep ones (three-plane SYRK 3.54 -> 3.91 ms)
+    static constexpr bool PRIO = TK == 64;
 };
 
 __device__ inline f4 mfma(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
@@ -140,12 +144,12 @@ __device__ __forceinline__ void nt_tile2(__bf16* __restrict__ lds, XA xaddr, siz
                     }
         }
     };
-    // the MFMA phase runs at raised wave priority, so that a wave ready to issue MFMAs goes before
-    // the other waves' loads and LDS stashes (2-3% shorter tiles on gfx950, results unchanged)
+    // with C::PRIO the MFMA phase runs at raised wave priority, so that a wave ready to issue MFMAs
+    // goes before the other waves' loads and LDS stashes (results unchanged)
     auto compute_prio = [&](int buf) {
-        __builtin_amdgcn_s_setprio(1);
+        if constexpr (C::PRIO) __builtin_amdgcn_s_setprio(1);
         compute(buf);
-        __builtin_amdgcn_s_setprio(0);
+        if constexpr (C::PRIO) __builtin_amdgcn_s_setprio(0);
     };
     if (nq <= 0) return;
     fetch(0);
